@@ -1,0 +1,675 @@
+// C ABI of the gfx950 frame loop (declarations and contracts in include/azu_hip.h).
+// Owns the handles, validates arguments, turns HIP failures into status codes and
+// thread-local messages, and optionally brackets every launch with HIP events for the
+// per-kernel timing report.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/azu_hip.h"
+#include "common.hpp"
+
+namespace azu {
+hipError_t launchIntegrate(int format, bool exact, void* vox, const VolParams& p, Intr K, int W, int H,
+                           const float* depth, float min_d, float max_d, const PoseArg& pa,
+                           unsigned long long* counters, hipStream_t st);
+hipError_t launchFillEmpty(int format, void* vox, size_t n, hipStream_t st);
+hipError_t launchRaycast(int format, bool exact, const void* vox, const VolParams& p, const PoseArg& pa, Intr K,
+                         int W, int H, float* verts, float* norms, float* t_hit, hipStream_t st);
+hipError_t launchLive(bool mm, const void* src, int W, int H, Intr K, float min_d, float max_d, float* const d[3],
+                      float* const v[3], float* const n[3], hipStream_t st);
+size_t icpStateBytes();
+int icpBlocksFor(int W, int H);
+size_t icpPartialsBytes(int maxBlocks);
+hipError_t launchIcpInit(void* st, const PoseArg& est, const PoseArg& ref, hipStream_t s);
+hipError_t launchIcpIter(const float* lv, const float* ln, int W, int H, const float* mv, const float* mn, int MW,
+                         int MH, Intr Km, float angle_cos, float dist_sq, int level, int iter, void* st,
+                         double* partials, unsigned* ticket, hipStream_t s);
+void icpStateRead(const void* hostCopy, float* pose, float* information, float* error, float* final_step,
+                  int* inliers, int* converged, int* counters, int* iterations);
+hipError_t launchVolConvert(int format, bool to_float, void* vox, float* tsdf, float* weight, size_t n,
+                            hipStream_t st);
+}  // namespace azu
+
+using namespace azu;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+int hipFail(hipError_t e, const char* what) {
+    return fail(AZU_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+#define AZU_HIP_TRY(expr, what)                          \
+    do {                                                 \
+        const hipError_t e_ = (expr);                    \
+        if (e_ != hipSuccess) return hipFail(e_, what);  \
+    } while (0)
+
+// ---- per-kernel timing ----------------------------------------------------------
+struct ProfEntry {
+    long long launches = 0;
+    double total_ms = 0.0;
+};
+struct ProfPending {
+    std::string name;
+    hipEvent_t a, b;
+};
+std::mutex g_prof_mu;
+bool g_prof_on = false;
+std::map<std::string, ProfEntry> g_prof;
+std::vector<ProfPending> g_pending;
+
+struct ProfScope {
+    hipStream_t s;
+    const char* name;
+    hipEvent_t a{}, b{};
+    bool on;
+    ProfScope(const char* n, hipStream_t st) : s(st), name(n) {
+        std::lock_guard<std::mutex> lk(g_prof_mu);
+        on = g_prof_on;
+        if (on) {
+            (void)hipEventCreate(&a);
+            (void)hipEventCreate(&b);
+            (void)hipEventRecord(a, s);
+        }
+    }
+    ~ProfScope() {
+        if (!on) return;
+        (void)hipEventRecord(b, s);
+        std::lock_guard<std::mutex> lk(g_prof_mu);
+        g_pending.push_back(ProfPending{name, a, b});
+    }
+};
+
+void profCollect() {
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    for (auto& p : g_pending) {
+        (void)hipEventSynchronize(p.b);
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            ProfEntry& e = g_prof[p.name];
+            e.launches += 1;
+            e.total_ms += ms;
+        }
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    g_pending.clear();
+}
+
+bool validParams(const azu_tsdf_params* p, std::string* why) {
+    if (!p) { *why = "null params"; return false; }
+    if (p->resolution < 2 || p->resolution > 2048) { *why = "resolution must be in [2, 2048]"; return false; }
+    if (!(p->voxel_size > 0.0f) || !std::isfinite(p->voxel_size)) { *why = "voxel_size must be > 0"; return false; }
+    if (!(p->truncation > 0.0f) || !std::isfinite(p->truncation)) { *why = "truncation must be > 0"; return false; }
+    if (!(p->max_weight >= 1.0f) || !(p->max_weight <= 65535.0f) || p->max_weight != std::floor(p->max_weight)) {
+        *why = "max_weight must be an integer in [1, 65535]";
+        return false;
+    }
+    for (float o : p->origin)
+        if (!std::isfinite(o)) { *why = "origin must be finite"; return false; }
+    return true;
+}
+
+VolParams toVol(const azu_tsdf_params& p) {
+    VolParams v;
+    v.res = p.resolution;
+    v.vs = p.voxel_size;
+    v.trunc = p.truncation;
+    v.max_w = p.max_weight;
+    v.origin[0] = p.origin[0];
+    v.origin[1] = p.origin[1];
+    v.origin[2] = p.origin[2];
+    v.min_depth = p.min_depth;
+    v.max_depth = p.max_depth;
+    return v;
+}
+
+bool sameParams(const azu_tsdf_params& a, const azu_tsdf_params& b) {
+    return a.resolution == b.resolution && a.voxel_size == b.voxel_size && a.truncation == b.truncation &&
+           a.max_weight == b.max_weight && a.min_depth == b.min_depth && a.max_depth == b.max_depth &&
+           a.origin[0] == b.origin[0] && a.origin[1] == b.origin[1] && a.origin[2] == b.origin[2];
+}
+
+PoseArg poseByValue(const float* m) {
+    PoseArg pa;
+    for (int i = 0; i < 16; ++i) pa.m[i] = m[i];
+    pa.dptr = nullptr;
+    return pa;
+}
+PoseArg poseByPtr(const float* d) {
+    PoseArg pa;
+    for (int i = 0; i < 16; ++i) pa.m[i] = 0.0f;
+    pa.dptr = d;
+    return pa;
+}
+bool finiteK(const float* K) {
+    return K && std::isfinite(K[0]) && std::isfinite(K[1]) && std::isfinite(K[2]) && std::isfinite(K[3]) &&
+           K[0] != 0.0f && K[1] != 0.0f;
+}
+
+// scaleIntrinsics (include/sensor/CameraIntrinsics.h:85-92)
+Intr scaleIntrinsics(Intr k, int rw, int rh, int w, int h) {
+    if (w == rw && h == rh) return k;
+    const float sx = (float)w / (float)rw, sy = (float)h / (float)rh;
+    return Intr{k.fx * sx, k.fy * sy, (k.cx + 0.5f) * sx - 0.5f, (k.cy + 0.5f) * sy - 0.5f};
+}
+
+}  // namespace
+
+struct azu_volume {
+    azu_tsdf_params params;
+    int format;
+    int mode;
+    void* vox = nullptr;
+    size_t bytes = 0;
+    unsigned long long* counters = nullptr;  // device [2]: updated, newly observed
+    int frames = 0;
+    bool exact() const { return mode == AZU_MODE_EXACT || (mode == AZU_MODE_AUTO && format == AZU_FMT_F32); }
+};
+
+struct azu_tracker {
+    int W, H;
+    Intr K;
+    azu_icp_params params;
+    int lw[3], lh[3];
+    float* d[3] = {nullptr, nullptr, nullptr};
+    float* v[3] = {nullptr, nullptr, nullptr};
+    float* n[3] = {nullptr, nullptr, nullptr};
+    void* state = nullptr;
+    double* partials = nullptr;
+    unsigned* ticket = nullptr;
+    void* host_state = nullptr;  // pinned
+    bool tracked = false;
+};
+
+namespace {
+azu_icp_params sanitize(const azu_icp_params& p) {
+    azu_icp_params o = p;
+    // sanitizeParams (src/tracking/ICPTracker.cpp:24-34)
+    if (!std::isfinite(o.angle_threshold)) o.angle_threshold = 30.0f;
+    else if (o.angle_threshold < 0.0f) o.angle_threshold = 0.0f;
+    else if (o.angle_threshold > 85.0f) o.angle_threshold = 85.0f;
+    return o;
+}
+
+int volAlloc(azu_volume* v) {
+    const size_t n = (size_t)v->params.resolution * v->params.resolution * v->params.resolution;
+    const size_t wb = v->format == AZU_FMT_F32 ? 8 : 4;
+    v->bytes = n * wb;
+    const hipError_t e = hipMalloc(&v->vox, v->bytes);
+    if (e != hipSuccess) {
+        v->vox = nullptr;
+        return fail(AZU_ERR_ALLOC, std::string("volume allocation of ") + std::to_string(v->bytes) +
+                                       " bytes failed: " + hipGetErrorString(e));
+    }
+    return AZU_OK;
+}
+}  // namespace
+
+extern "C" {
+
+const char* azu_last_error(void) { return g_err.c_str(); }
+int azu_abi_version(void) { return AZU_HIP_ABI_VERSION; }
+
+int azu_device_info(int* device, char* name, size_t name_len, size_t* hbm_bytes) {
+    int dev = 0;
+    AZU_HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
+    hipDeviceProp_t prop;
+    AZU_HIP_TRY(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+    if (device) *device = dev;
+    if (name && name_len) {
+        std::snprintf(name, name_len, "%s (%s)", prop.name, prop.gcnArchName);
+    }
+    if (hbm_bytes) *hbm_bytes = prop.totalGlobalMem;
+    return AZU_OK;
+}
+
+int azu_malloc(void** p, size_t bytes) {
+    if (!p) return fail(AZU_ERR_ARG, "null output pointer");
+    const hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) return fail(AZU_ERR_ALLOC, std::string("hipMalloc: ") + hipGetErrorString(e));
+    return AZU_OK;
+}
+int azu_free(void* p) {
+    AZU_HIP_TRY(hipFree(p), "hipFree");
+    return AZU_OK;
+}
+int azu_host_alloc(void** p, size_t bytes) {
+    if (!p) return fail(AZU_ERR_ARG, "null output pointer");
+    const hipError_t e = hipHostMalloc(p, bytes, hipHostMallocDefault);
+    if (e != hipSuccess) return fail(AZU_ERR_ALLOC, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+    return AZU_OK;
+}
+int azu_host_free(void* p) {
+    AZU_HIP_TRY(hipHostFree(p), "hipHostFree");
+    return AZU_OK;
+}
+int azu_memcpy_h2d(void* d, const void* s, size_t b, azu_stream st) {
+    AZU_HIP_TRY(hipMemcpyAsync(d, s, b, hipMemcpyHostToDevice, (hipStream_t)st), "hipMemcpyAsync H2D");
+    return AZU_OK;
+}
+int azu_memcpy_d2h(void* d, const void* s, size_t b, azu_stream st) {
+    AZU_HIP_TRY(hipMemcpyAsync(d, s, b, hipMemcpyDeviceToHost, (hipStream_t)st), "hipMemcpyAsync D2H");
+    return AZU_OK;
+}
+int azu_memcpy_d2d(void* d, const void* s, size_t b, azu_stream st) {
+    AZU_HIP_TRY(hipMemcpyAsync(d, s, b, hipMemcpyDeviceToDevice, (hipStream_t)st), "hipMemcpyAsync D2D");
+    return AZU_OK;
+}
+int azu_stream_create(azu_stream* s) {
+    if (!s) return fail(AZU_ERR_ARG, "null output pointer");
+    hipStream_t h;
+    AZU_HIP_TRY(hipStreamCreateWithFlags(&h, hipStreamNonBlocking), "hipStreamCreate");
+    *s = h;
+    return AZU_OK;
+}
+int azu_stream_destroy(azu_stream s) {
+    AZU_HIP_TRY(hipStreamDestroy((hipStream_t)s), "hipStreamDestroy");
+    return AZU_OK;
+}
+int azu_stream_sync(azu_stream s) {
+    AZU_HIP_TRY(hipStreamSynchronize((hipStream_t)s), "hipStreamSynchronize");
+    return AZU_OK;
+}
+
+// ---- volume ----------------------------------------------------------------------
+int azu_vol_create(const azu_tsdf_params* p, int format, azu_volume** out) {
+    if (!out) return fail(AZU_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    std::string why;
+    if (!validParams(p, &why)) return fail(AZU_ERR_ARG, why);
+    if (format != AZU_FMT_F32 && format != AZU_FMT_F16_U16) return fail(AZU_ERR_ARG, "unknown volume format");
+    auto* v = new azu_volume();
+    v->params = *p;
+    v->format = format;
+    v->mode = AZU_MODE_AUTO;
+    int rc = volAlloc(v);
+    if (rc != AZU_OK) {
+        delete v;
+        return rc;
+    }
+    if (hipMalloc(&v->counters, 2 * sizeof(unsigned long long)) != hipSuccess) {
+        (void)hipFree(v->vox);
+        delete v;
+        return fail(AZU_ERR_ALLOC, "counter allocation failed");
+    }
+    rc = azu_vol_reset(v, nullptr);
+    if (rc == AZU_OK) {
+        const hipError_t e = hipStreamSynchronize(nullptr);
+        if (e != hipSuccess) rc = hipFail(e, "volume reset");
+    }
+    if (rc != AZU_OK) {
+        azu_vol_destroy(v);
+        return rc;
+    }
+    *out = v;
+    return AZU_OK;
+}
+
+int azu_vol_destroy(azu_volume* v) {
+    if (!v) return AZU_OK;
+    if (v->vox) (void)hipFree(v->vox);
+    if (v->counters) (void)hipFree(v->counters);
+    delete v;
+    return AZU_OK;
+}
+
+int azu_vol_reset(azu_volume* v, azu_stream s) {
+    if (!v) return fail(AZU_ERR_ARG, "null volume");
+    const size_t n = (size_t)v->params.resolution * v->params.resolution * v->params.resolution;
+    {
+        ProfScope ps("fill_empty", (hipStream_t)s);
+        AZU_HIP_TRY(launchFillEmpty(v->format, v->vox, n, (hipStream_t)s), "fill_empty launch");
+    }
+    AZU_HIP_TRY(hipMemsetAsync(v->counters, 0, 2 * sizeof(unsigned long long), (hipStream_t)s), "counter reset");
+    v->frames = 0;
+    return AZU_OK;
+}
+
+int azu_vol_set_params(azu_volume* v, const azu_tsdf_params* p) {
+    if (!v) return fail(AZU_ERR_ARG, "null volume");
+    std::string why;
+    if (!validParams(p, &why)) return fail(AZU_ERR_ARG, why);
+    if (sameParams(*p, v->params)) return AZU_OK;
+    AZU_HIP_TRY(hipDeviceSynchronize(), "synchronise before setParams");
+    const bool realloc_ = p->resolution != v->params.resolution;
+    v->params = *p;
+    if (realloc_) {
+        (void)hipFree(v->vox);
+        v->vox = nullptr;
+        const int rc = volAlloc(v);
+        if (rc != AZU_OK) return rc;
+    }
+    const int rc = azu_vol_reset(v, nullptr);
+    if (rc != AZU_OK) return rc;
+    AZU_HIP_TRY(hipStreamSynchronize(nullptr), "volume reset");
+    return AZU_OK;
+}
+
+int azu_vol_get_params(const azu_volume* v, azu_tsdf_params* p, int* format) {
+    if (!v) return fail(AZU_ERR_ARG, "null volume");
+    if (p) *p = v->params;
+    if (format) *format = v->format;
+    return AZU_OK;
+}
+
+int azu_vol_set_mode(azu_volume* v, int mode) {
+    if (!v) return fail(AZU_ERR_ARG, "null volume");
+    if (mode != AZU_MODE_AUTO && mode != AZU_MODE_EXACT && mode != AZU_MODE_FAST)
+        return fail(AZU_ERR_ARG, "unknown kernel mode");
+    if (mode == AZU_MODE_FAST && v->params.resolution % 4 != 0)
+        return fail(AZU_ERR_ARG, "the fast kernels need a resolution divisible by 4");
+    v->mode = mode;
+    return AZU_OK;
+}
+
+static int volTransfer(azu_volume* v, float* tsdf, float* weight, bool download) {
+    if (!v || !tsdf || !weight) return fail(AZU_ERR_ARG, "null argument");
+    const size_t n = (size_t)v->params.resolution * v->params.resolution * v->params.resolution;
+    const size_t chunk = std::min<size_t>(n, (size_t)1 << 24);
+    float *dt = nullptr, *dw = nullptr;
+    if (hipMalloc(&dt, chunk * 4) != hipSuccess || hipMalloc(&dw, chunk * 4) != hipSuccess) {
+        if (dt) (void)hipFree(dt);
+        return fail(AZU_ERR_ALLOC, "staging allocation failed");
+    }
+    int rc = AZU_OK;
+    const size_t wb = v->format == AZU_FMT_F32 ? 8 : 4;
+    for (size_t off = 0; off < n && rc == AZU_OK; off += chunk) {
+        const size_t m = std::min(chunk, n - off);
+        char* base = static_cast<char*>(v->vox) + off * wb;
+        hipError_t e = hipSuccess;
+        if (download) {
+            e = launchVolConvert(v->format, true, base, dt, dw, m, nullptr);
+            if (e == hipSuccess) e = hipMemcpy(tsdf + off, dt, m * 4, hipMemcpyDeviceToHost);
+            if (e == hipSuccess) e = hipMemcpy(weight + off, dw, m * 4, hipMemcpyDeviceToHost);
+        } else {
+            e = hipMemcpy(dt, tsdf + off, m * 4, hipMemcpyHostToDevice);
+            if (e == hipSuccess) e = hipMemcpy(dw, weight + off, m * 4, hipMemcpyHostToDevice);
+            if (e == hipSuccess) e = launchVolConvert(v->format, false, base, dt, dw, m, nullptr);
+            if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+        }
+        if (e != hipSuccess) rc = hipFail(e, download ? "volume download" : "volume upload");
+    }
+    (void)hipFree(dt);
+    (void)hipFree(dw);
+    return rc;
+}
+
+int azu_vol_download(azu_volume* v, float* tsdf, float* weight) { return volTransfer(v, tsdf, weight, true); }
+int azu_vol_upload(azu_volume* v, const float* tsdf, const float* weight) {
+    return volTransfer(v, const_cast<float*>(tsdf), const_cast<float*>(weight), false);
+}
+
+int azu_vol_storage(azu_volume* v, void** d_ptr, size_t* bytes) {
+    if (!v) return fail(AZU_ERR_ARG, "null volume");
+    if (d_ptr) *d_ptr = v->vox;
+    if (bytes) *bytes = v->bytes;
+    return AZU_OK;
+}
+
+int azu_vol_counters(azu_volume* v, unsigned long long* updated, unsigned long long* newly, int reset) {
+    if (!v) return fail(AZU_ERR_ARG, "null volume");
+    unsigned long long h[2] = {0, 0};
+    AZU_HIP_TRY(hipDeviceSynchronize(), "synchronise for counters");
+    AZU_HIP_TRY(hipMemcpy(h, v->counters, sizeof(h), hipMemcpyDeviceToHost), "counter read");
+    if (updated) *updated = h[0];
+    if (newly) *newly = h[1];
+    if (reset) AZU_HIP_TRY(hipMemset(v->counters, 0, sizeof(h)), "counter reset");
+    return AZU_OK;
+}
+
+int azu_vol_integrated_frames(const azu_volume* v, int* frames) {
+    if (!v || !frames) return fail(AZU_ERR_ARG, "null argument");
+    *frames = v->frames;
+    return AZU_OK;
+}
+
+static int integrateImpl(azu_volume* v, const float* d_depth, int W, int H, const float* K, const PoseArg& pa,
+                         float min_d, float max_d, azu_stream s) {
+    if (!v || !d_depth) return fail(AZU_ERR_ARG, "null volume or depth");
+    if (W <= 0 || H <= 0) return fail(AZU_ERR_ARG, "image size must be positive");
+    if (!finiteK(K)) return fail(AZU_ERR_ARG, "intrinsics must be finite with non-zero focal lengths");
+    const bool exact = v->exact();
+    if (!exact && v->params.resolution % 4 != 0)
+        return fail(AZU_ERR_ARG, "the fast kernels need a resolution divisible by 4");
+    ProfScope ps(exact ? (v->format == AZU_FMT_F32 ? "integrate_exact_f32" : "integrate_exact_f16")
+                       : (v->format == AZU_FMT_F32 ? "integrate_fast_f32" : "integrate_fast_f16"),
+                 (hipStream_t)s);
+    AZU_HIP_TRY(launchIntegrate(v->format, exact, v->vox, toVol(v->params), Intr{K[0], K[1], K[2], K[3]}, W, H,
+                                d_depth, min_d, max_d, pa, v->counters, (hipStream_t)s),
+                "integrate launch");
+    v->frames += 1;
+    return AZU_OK;
+}
+
+int azu_integrate(azu_volume* v, const float* d_depth_m, int W, int H, const float K[4], const float pose[16],
+                  float min_depth, float max_depth, azu_stream s) {
+    if (!pose) return fail(AZU_ERR_ARG, "null pose");
+    return integrateImpl(v, d_depth_m, W, H, K, poseByValue(pose), min_depth, max_depth, s);
+}
+
+int azu_integrate_dpose(azu_volume* v, const float* d_depth_m, int W, int H, const float K[4], const float* d_pose,
+                        float min_depth, float max_depth, azu_stream s) {
+    if (!d_pose) return fail(AZU_ERR_ARG, "null device pose");
+    return integrateImpl(v, d_depth_m, W, H, K, poseByPtr(d_pose), min_depth, max_depth, s);
+}
+
+static int raycastImpl(azu_volume* v, const PoseArg& pa, const float* K, int W, int H, float* dv, float* dn,
+                       float* dt, azu_stream s) {
+    if (!v || !dv || !dn) return fail(AZU_ERR_ARG, "null volume or output");
+    if (W <= 0 || H <= 0) return fail(AZU_ERR_ARG, "image size must be positive");
+    if (!finiteK(K)) return fail(AZU_ERR_ARG, "intrinsics must be finite with non-zero focal lengths");
+    const bool exact = v->exact();
+    ProfScope ps(exact ? (v->format == AZU_FMT_F32 ? "raycast_exact_f32" : "raycast_exact_f16")
+                       : (v->format == AZU_FMT_F32 ? "raycast_fast_f32" : "raycast_fast_f16"),
+                 (hipStream_t)s);
+    AZU_HIP_TRY(launchRaycast(v->format, exact, v->vox, toVol(v->params), pa, Intr{K[0], K[1], K[2], K[3]}, W, H,
+                              dv, dn, dt, (hipStream_t)s),
+                "raycast launch");
+    return AZU_OK;
+}
+
+int azu_raycast(azu_volume* v, const float pose[16], const float K[4], int W, int H, float* d_vertices,
+                float* d_normals, azu_stream s) {
+    if (!pose) return fail(AZU_ERR_ARG, "null pose");
+    return raycastImpl(v, poseByValue(pose), K, W, H, d_vertices, d_normals, nullptr, s);
+}
+
+int azu_raycast_dpose(azu_volume* v, const float* d_pose, const float K[4], int W, int H, float* d_vertices,
+                      float* d_normals, float* d_t_hit, azu_stream s) {
+    if (!d_pose) return fail(AZU_ERR_ARG, "null device pose");
+    return raycastImpl(v, poseByPtr(d_pose), K, W, H, d_vertices, d_normals, d_t_hit, s);
+}
+
+// ---- tracker ---------------------------------------------------------------------
+int azu_tracker_destroy(azu_tracker* t) {
+    if (!t) return AZU_OK;
+    for (int l = 0; l < 3; ++l) {
+        if (t->d[l]) (void)hipFree(t->d[l]);
+        if (t->v[l]) (void)hipFree(t->v[l]);
+        if (t->n[l]) (void)hipFree(t->n[l]);
+    }
+    if (t->state) (void)hipFree(t->state);
+    if (t->partials) (void)hipFree(t->partials);
+    if (t->ticket) (void)hipFree(t->ticket);
+    if (t->host_state) (void)hipHostFree(t->host_state);
+    delete t;
+    return AZU_OK;
+}
+
+int azu_tracker_create(int W, int H, const float K[4], const azu_icp_params* p, azu_tracker** out) {
+    if (!out) return fail(AZU_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    if (W < 4 || H < 4 || W > 8192 || H > 8192) return fail(AZU_ERR_ARG, "live frame size must be in [4, 8192]");
+    if (!finiteK(K)) return fail(AZU_ERR_ARG, "intrinsics must be finite with non-zero focal lengths");
+    if (!p) return fail(AZU_ERR_ARG, "null ICP params");
+    auto* t = new azu_tracker();
+    t->W = W;
+    t->H = H;
+    t->K = Intr{K[0], K[1], K[2], K[3]};
+    t->params = sanitize(*p);
+    t->lw[0] = W;
+    t->lh[0] = H;
+    for (int l = 1; l < 3; ++l) {
+        t->lw[l] = t->lw[l - 1] / 2;
+        t->lh[l] = t->lh[l - 1] / 2;
+    }
+    bool ok = true;
+    for (int l = 0; l < 3 && ok; ++l) {
+        const size_t np = (size_t)t->lw[l] * t->lh[l];
+        ok = hipMalloc(&t->d[l], np * 4) == hipSuccess && hipMalloc(&t->v[l], np * 12) == hipSuccess &&
+             hipMalloc(&t->n[l], np * 12) == hipSuccess;
+    }
+    ok = ok && hipMalloc(&t->state, icpStateBytes()) == hipSuccess;
+    ok = ok && hipMalloc((void**)&t->partials, icpPartialsBytes(1024)) == hipSuccess;
+    ok = ok && hipMalloc((void**)&t->ticket, sizeof(unsigned)) == hipSuccess;
+    ok = ok && hipHostMalloc(&t->host_state, icpStateBytes(), hipHostMallocDefault) == hipSuccess;
+    if (!ok) {
+        azu_tracker_destroy(t);
+        return fail(AZU_ERR_ALLOC, "tracker allocation failed");
+    }
+    if (hipMemset(t->ticket, 0, sizeof(unsigned)) != hipSuccess ||
+        hipMemset(t->state, 0, icpStateBytes()) != hipSuccess) {
+        azu_tracker_destroy(t);
+        return fail(AZU_ERR_HIP, "tracker initialisation failed");
+    }
+    *out = t;
+    return AZU_OK;
+}
+
+int azu_tracker_set_params(azu_tracker* t, const azu_icp_params* p) {
+    if (!t || !p) return fail(AZU_ERR_ARG, "null argument");
+    t->params = sanitize(*p);
+    return AZU_OK;
+}
+
+static int prepareImpl(azu_tracker* t, const void* src, bool mm, azu_stream s) {
+    if (!t || !src) return fail(AZU_ERR_ARG, "null tracker or depth");
+    ProfScope ps("live_pyramid", (hipStream_t)s);
+    AZU_HIP_TRY(launchLive(mm, src, t->W, t->H, t->K, t->params.min_depth, t->params.max_depth, t->d, t->v, t->n,
+                           (hipStream_t)s),
+                "live pyramid launch");
+    return AZU_OK;
+}
+int azu_tracker_prepare_mm(azu_tracker* t, const uint16_t* d, azu_stream s) { return prepareImpl(t, d, true, s); }
+int azu_tracker_prepare_m(azu_tracker* t, const float* d, azu_stream s) { return prepareImpl(t, d, false, s); }
+
+int azu_tracker_level(azu_tracker* t, int level, const float** d, const float** v, const float** n, int* W, int* H) {
+    if (!t || level < 0 || level > 2) return fail(AZU_ERR_ARG, "bad tracker or level");
+    if (d) *d = t->d[level];
+    if (v) *v = t->v[level];
+    if (n) *n = t->n[level];
+    if (W) *W = t->lw[level];
+    if (H) *H = t->lh[level];
+    return AZU_OK;
+}
+
+static int trackImpl(azu_tracker* t, const float* mv, const float* mn, int MW, int MH, const PoseArg& est,
+                     const PoseArg& ref, azu_stream s) {
+    if (!t || !mv || !mn) return fail(AZU_ERR_ARG, "null tracker or model");
+    if (MW <= 0 || MH <= 0) return fail(AZU_ERR_ARG, "model size must be positive");
+    const hipStream_t st = (hipStream_t)s;
+    const Intr Km = scaleIntrinsics(t->K, t->W, t->H, MW, MH);
+    // cos(angle * pi / 180) evaluated in double and rounded to float (ICPTracker.cpp:160)
+    const float angle_cos = (float)std::cos(t->params.angle_threshold * M_PI / 180.0f);
+    const float dist_sq = t->params.dist_threshold * t->params.dist_threshold;
+    {
+        ProfScope ps("icp_init", st);
+        AZU_HIP_TRY(launchIcpInit(t->state, est, ref, st), "ICP init launch");
+    }
+    for (int level = 2; level >= 0; --level) {
+        const int iters = t->params.max_iterations[level];
+        for (int it = 0; it < iters; ++it) {
+            ProfScope ps("icp_iter", st);
+            AZU_HIP_TRY(launchIcpIter(t->v[level], t->n[level], t->lw[level], t->lh[level], mv, mn, MW, MH, Km,
+                                      angle_cos, dist_sq, level, it, t->state, t->partials, t->ticket, st),
+                        "ICP iteration launch");
+        }
+    }
+    t->tracked = true;
+    return AZU_OK;
+}
+
+int azu_tracker_track(azu_tracker* t, const float* mv, const float* mn, int MW, int MH, const float est[16],
+                      const float ref[16], azu_stream s) {
+    if (!est || !ref) return fail(AZU_ERR_ARG, "null pose");
+    return trackImpl(t, mv, mn, MW, MH, poseByValue(est), poseByValue(ref), s);
+}
+
+int azu_tracker_track_dpose(azu_tracker* t, const float* mv, const float* mn, int MW, int MH, const float* d_est,
+                            const float* d_ref, azu_stream s) {
+    if (!d_est || !d_ref) return fail(AZU_ERR_ARG, "null device pose");
+    return trackImpl(t, mv, mn, MW, MH, poseByPtr(d_est), poseByPtr(d_ref), s);
+}
+
+int azu_tracker_pose_device(azu_tracker* t, const float** d_pose) {
+    if (!t || !d_pose) return fail(AZU_ERR_ARG, "null argument");
+    *d_pose = static_cast<const float*>(t->state);  // IcpState::pose is its first member
+    return AZU_OK;
+}
+
+int azu_tracker_result(azu_tracker* t, azu_icp_result* out, azu_stream s) {
+    if (!t || !out) return fail(AZU_ERR_ARG, "null argument");
+    if (!t->tracked) return fail(AZU_ERR_STATE, "no track has been enqueued");
+    AZU_HIP_TRY(hipMemcpyAsync(t->host_state, t->state, icpStateBytes(), hipMemcpyDeviceToHost, (hipStream_t)s),
+                "state read");
+    AZU_HIP_TRY(hipStreamSynchronize((hipStream_t)s), "state read sync");
+    int counters[5];
+    icpStateRead(t->host_state, out->pose, out->information, &out->error, &out->final_step, &out->inliers,
+                 &out->converged, counters, &out->iterations);
+    out->valid_live_points = counters[0];
+    out->valid_model_points = counters[1];
+    out->projected_points = counters[2];
+    out->dist_filtered = counters[3];
+    out->angle_filtered = counters[4];
+    bool fin = true;
+    for (float x : out->pose) fin = fin && std::isfinite(x);
+    // ICPTracker::track's success rule (ICPTracker.cpp:62-64)
+    out->tracking_ok = (fin && out->inliers > 100 && (out->converged || out->final_step <= 1e-3f)) ? 1 : 0;
+    return AZU_OK;
+}
+
+// ---- profiling -------------------------------------------------------------------
+int azu_profile_enable(int on) {
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    g_prof_on = on != 0;
+    return AZU_OK;
+}
+int azu_profile_reset(void) {
+    profCollect();
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    g_prof.clear();
+    return AZU_OK;
+}
+int azu_profile_read(char* names, int n_max, long long* launches, double* total_ms, int* n) {
+    profCollect();
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    int i = 0;
+    for (const auto& kv : g_prof) {
+        if (i >= n_max) break;
+        if (names) {
+            std::memset(names + 48 * i, 0, 48);
+            std::strncpy(names + 48 * i, kv.first.c_str(), 47);
+        }
+        if (launches) launches[i] = kv.second.launches;
+        if (total_ms) total_ms[i] = kv.second.total_ms;
+        ++i;
+    }
+    if (n) *n = i;
+    return AZU_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,546 @@
+// Projective point-to-plane ICP on gfx950 (the reference's ICPTracker::track /
+// trackLevel / buildLinearSystem, src/tracking/ICPTracker.cpp:40-354, with the
+// constants and predicates of include/tracking/ICPShared.h:34-110).
+//
+// One kernel launch per ICP iteration, no host round trip:
+//   1. every thread associates its live pixels with the model image, builds
+//      J = [R^T n_m ; v x R^T n_m], the Huber weight and loss, and accumulates the
+//      21 upper-triangle A terms, 6 b terms and the residual in FP32 with Kahan
+//      (TwoSum) compensation, plus 6 integer counters;
+//   2. the block folds its lanes with wave shuffles and LDS, still as compensated
+//      FP32 pairs, and writes one FP64 partial per slot;
+//   3. the last block to finish (atomic ticket, agent-scope fences) sums the partials
+//      in FP64 in block order (deterministic), and one lane runs the solve: adaptive
+//      damping from the extreme eigenvalues, FP64 Cholesky of A + lambda I, step caps,
+//      Rodrigues update, pose * delta, re-projection onto SO(3), convergence test.
+// The tracker state (pose, per-level result, stop flag) lives in device memory, so
+// all 19 iterations of a frame are enqueued back to back; once a level has stopped,
+// its remaining launches return at once.
+#include "common.hpp"
+
+namespace azu {
+
+constexpr int kIcpSlotsF = 28;  // 21 A + 6 b + residual
+constexpr int kIcpSlotsI = 6;   // inliers, valid_live, valid_model, projected, dist_f, angle_f
+constexpr int kIcpSlots = kIcpSlotsF + kIcpSlotsI;
+
+struct IcpState {
+    float pose[16];
+    float ref[16];
+    float information[36];
+    float error;
+    float final_step;       // last accepted step of the current level (inf if none)
+    float last_final_step;  // last accepted step over all levels (inf if none)
+    int inliers;
+    int converged;
+    int counters[5];  // valid_live, valid_model, projected, dist_filtered, angle_filtered
+    int stop_level;   // level whose iteration loop has ended, -1 while none
+    int iterations;   // solves run (diagnostic)
+};
+
+struct IcpIterArgs {
+    const float* lv;  // live level vertices / normals, packed xyz
+    const float* ln;
+    int W, H;
+    const float* mv;  // model image vertices / normals (world), packed xyz
+    const float* mn;
+    int MW, MH;
+    Intr Km;          // intrinsics scaled to the model image
+    float angle_cos;  // cos(angle threshold), rounded to float like the reference
+    float dist_sq;    // dist_threshold^2 in float
+    int level, iter;
+};
+
+__device__ __forceinline__ void twoSum(float& hi, float& lo, float b_hi, float b_lo) {
+    const float s = hi + b_hi;
+    const float bp = s - hi;
+    const float e = (hi - (s - bp)) + (b_hi - bp);
+    hi = s;
+    lo = lo + b_lo + e;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 small dense algebra for the single-lane solve.
+// ---------------------------------------------------------------------------
+__device__ bool cholesky6(const double* A, double* L) {
+    for (int j = 0; j < 6; ++j) {
+        double s = A[j * 6 + j];
+        for (int k = 0; k < j; ++k) s -= L[j * 6 + k] * L[j * 6 + k];
+        if (!(s > 0.0)) return false;
+        const double d = sqrt(s);
+        L[j * 6 + j] = d;
+        for (int i = j + 1; i < 6; ++i) {
+            double t = A[i * 6 + j];
+            for (int k = 0; k < j; ++k) t -= L[i * 6 + k] * L[j * 6 + k];
+            L[i * 6 + j] = t / d;
+        }
+    }
+    return true;
+}
+
+// Cyclic Jacobi eigenvalues (Golub & Van Loan 8.5) of a symmetric 6x6; returns min, max.
+__device__ void jacobiMinMax(const double* Ain, double& mn, double& mx) {
+    double a[36];
+    for (int i = 0; i < 36; ++i) a[i] = Ain[i];
+    for (int sweep = 0; sweep < 64; ++sweep) {
+        double off = 0.0, diag = 0.0;
+        for (int i = 0; i < 6; ++i) {
+            diag += a[i * 6 + i] * a[i * 6 + i];
+            for (int j = i + 1; j < 6; ++j) off += a[i * 6 + j] * a[i * 6 + j];
+        }
+        if (off == 0.0 || off <= 1e-32 * diag) break;
+        for (int p = 0; p < 6; ++p)
+            for (int q = p + 1; q < 6; ++q) {
+                const double apq = a[p * 6 + q];
+                if (apq == 0.0) continue;
+                const double theta = (a[q * 6 + q] - a[p * 6 + p]) / (2.0 * apq);
+                const double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+                for (int k = 0; k < 6; ++k) {
+                    const double akp = a[k * 6 + p], akq = a[k * 6 + q];
+                    a[k * 6 + p] = c * akp - s * akq;
+                    a[k * 6 + q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < 6; ++k) {
+                    const double apk = a[p * 6 + k], aqk = a[q * 6 + k];
+                    a[p * 6 + k] = c * apk - s * aqk;
+                    a[q * 6 + k] = s * apk + c * aqk;
+                }
+            }
+    }
+    mn = a[0];
+    mx = a[0];
+    for (int i = 1; i < 6; ++i) {
+        mn = fmin(mn, a[i * 6 + i]);
+        mx = fmax(mx, a[i * 6 + i]);
+    }
+}
+
+// dampingForHessian (ICPShared.h:293-307): escalate to 1.0 when min_eig < 1e-4 or
+// max_eig > 1e7 min_eig. A Cholesky certificate settles the common case without the
+// eigen-solve: with g >= max_eig (Gershgorin) and s = max(1e-4, g / 1e7), a successful
+// factorisation of A - s I proves min_eig > s, hence neither escalation condition.
+__device__ double dampingFP64(const double* A, int level) {
+    const double kMin = (double)1e-4f, kCond = (double)1e7f;
+    const double base = level == 0 ? (double)0.01f : (double)0.1f;
+    double g = 0.0;
+    bool fin = true;
+    for (int i = 0; i < 6; ++i) {
+        double r = A[i * 6 + i];
+        for (int j = 0; j < 6; ++j)
+            if (j != i) r += fabs(A[i * 6 + j]);
+        g = fmax(g, r);
+        for (int j = 0; j < 6; ++j) fin = fin && isfinite(A[i * 6 + j]);
+    }
+    if (!fin) return 1.0;
+    const double s = fmax(kMin, g / kCond) * (1.0 + 1e-12);
+    double M[36], L[36];
+    for (int i = 0; i < 36; ++i) M[i] = A[i];
+    for (int i = 0; i < 6; ++i) M[i * 6 + i] -= s;
+    if (cholesky6(M, L)) return base;
+    double mn, mx;
+    jacobiMinMax(A, mn, mx);
+    if (!isfinite(mn) || !isfinite(mx) || mn < kMin) return 1.0;
+    if (mx > kCond * mn) return 1.0;
+    return base;
+}
+
+// Nearest rotation U V^T by one-sided Jacobi SVD in FP64 (projectToSO3,
+// ICPShared.h:309-317), negating U's smallest-singular-value column if det < 0.
+__device__ void projectSO3(const double* M, double* R) {
+    double U[9], V[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    for (int i = 0; i < 9; ++i) U[i] = M[i];
+    for (int sweep = 0; sweep < 60; ++sweep) {
+        double offmax = 0.0;
+        for (int p = 0; p < 2; ++p)
+            for (int q = p + 1; q < 3; ++q) {
+                double al = 0.0, be = 0.0, ga = 0.0;
+                for (int k = 0; k < 3; ++k) {
+                    al += U[k * 3 + p] * U[k * 3 + p];
+                    be += U[k * 3 + q] * U[k * 3 + q];
+                    ga += U[k * 3 + p] * U[k * 3 + q];
+                }
+                const double den = sqrt(al * be);
+                if (den == 0.0) continue;
+                offmax = fmax(offmax, fabs(ga) / den);
+                if (fabs(ga) <= 1e-300) continue;
+                const double ze = (be - al) / (2.0 * ga);
+                const double t = (ze >= 0.0 ? 1.0 : -1.0) / (fabs(ze) + sqrt(1.0 + ze * ze));
+                const double c = 1.0 / sqrt(1.0 + t * t), s = c * t;
+                for (int k = 0; k < 3; ++k) {
+                    const double up = U[k * 3 + p], uq = U[k * 3 + q];
+                    U[k * 3 + p] = c * up - s * uq;
+                    U[k * 3 + q] = s * up + c * uq;
+                    const double vp = V[k * 3 + p], vq = V[k * 3 + q];
+                    V[k * 3 + p] = c * vp - s * vq;
+                    V[k * 3 + q] = s * vp + c * vq;
+                }
+            }
+        if (offmax < 1e-15) break;
+    }
+    double sig[3];
+    for (int j = 0; j < 3; ++j) {
+        sig[j] = sqrt(U[j] * U[j] + U[3 + j] * U[3 + j] + U[6 + j] * U[6 + j]);
+        if (sig[j] > 0.0)
+            for (int k = 0; k < 3; ++k) U[k * 3 + j] /= sig[j];
+    }
+    for (int j = 0; j < 3; ++j) {
+        if (sig[j] > 0.0) continue;
+        const int a = (j + 1) % 3, b = (j + 2) % 3;
+        const double ca[3] = {U[a], U[3 + a], U[6 + a]}, cb[3] = {U[b], U[3 + b], U[6 + b]};
+        const double c[3] = {ca[1] * cb[2] - ca[2] * cb[1], ca[2] * cb[0] - ca[0] * cb[2], ca[0] * cb[1] - ca[1] * cb[0]};
+        const double l = sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+        for (int k = 0; k < 3; ++k) U[k * 3 + j] = l > 0.0 ? c[k] / l : (k == j ? 1.0 : 0.0);
+    }
+    for (int pass = 0; pass < 2; ++pass) {
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                R[i * 3 + j] = U[i * 3 + 0] * V[j * 3 + 0] + U[i * 3 + 1] * V[j * 3 + 1] + U[i * 3 + 2] * V[j * 3 + 2];
+        const double det = R[0] * (R[4] * R[8] - R[5] * R[7]) - R[1] * (R[3] * R[8] - R[5] * R[6]) +
+                           R[2] * (R[3] * R[7] - R[4] * R[6]);
+        if (pass == 1 || !(det < 0.0)) break;
+        int js = 0;
+        for (int j = 1; j < 3; ++j)
+            if (sig[j] < sig[js]) js = j;
+        for (int k = 0; k < 3; ++k) U[k * 3 + js] = -U[k * 3 + js];
+    }
+}
+
+// One solve + pose update (trackLevel's loop body, ICPTracker.cpp:79-130), lane 0 of
+// the last block. sums: 28 FP64 slots (21 A upper, 6 b, residual) + 6 counters.
+__device__ void icpSolve(IcpState* st, const double* sums, int level, int iter) {
+    if (iter == 0) {
+        st->error = 0.0f;
+        st->inliers = 0;
+        st->converged = 0;
+        st->final_step = __builtin_inff();
+        for (int i = 0; i < 36; ++i) st->information[i] = 0.0f;
+    }
+    const int inliers = (int)sums[kIcpSlotsF + 0];
+    for (int i = 0; i < 5; ++i) st->counters[i] = (int)sums[kIcpSlotsF + 1 + i];
+    if (inliers < 10) {  // covers buildLinearSystem's "no inlier" failure too
+        st->stop_level = level;
+        return;
+    }
+    st->iterations += 1;
+    double A[36], b[6];
+    int k = 0;
+    for (int i = 0; i < 6; ++i)
+        for (int j = i; j < 6; ++j) {
+            A[i * 6 + j] = sums[k];
+            A[j * 6 + i] = sums[k];
+            ++k;
+        }
+    for (int i = 0; i < 6; ++i) b[i] = sums[21 + i];
+    const double residual = sums[27];
+    for (int i = 0; i < 36; ++i) st->information[i] = (float)A[i];
+    const double lam = dampingFP64(A, level);
+    for (int i = 0; i < 6; ++i) A[i * 6 + i] += lam;
+    double L[36];
+    if (!cholesky6(A, L)) {
+        st->stop_level = level;
+        return;
+    }
+    double y[6], xd[6];
+    for (int i = 0; i < 6; ++i) {
+        double s = b[i];
+        for (int j = 0; j < i; ++j) s -= L[i * 6 + j] * y[j];
+        y[i] = s / L[i * 6 + i];
+    }
+    for (int i = 5; i >= 0; --i) {
+        double s = y[i];
+        for (int j = i + 1; j < 6; ++j) s -= L[j * 6 + i] * xd[j];
+        xd[i] = s / L[i * 6 + i];
+    }
+    float x[6];
+    bool fin = true;
+    for (int i = 0; i < 6; ++i) {
+        x[i] = (float)xd[i];
+        fin = fin && isfinite(x[i]);
+    }
+    if (!fin) {
+        st->stop_level = level;
+        return;
+    }
+    const float tn = sqrtf(sum3(x[0] * x[0], x[1] * x[1], x[2] * x[2]));
+    const float rn = sqrtf(sum3(x[3] * x[3], x[4] * x[4], x[5] * x[5]));
+    if (tn > 0.2f || rn > 0.5f) {
+        st->stop_level = level;
+        return;
+    }
+    float sq = 0.0f;
+    for (int i = 0; i < 6; ++i) sq += x[i] * x[i];
+    const float step_norm = sqrtf(sq);
+    const float rx = x[3], ry = x[4], rz = x[5];
+    float Rd[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    const float angle = sqrtf(rx * rx + ry * ry + rz * rz);
+    if (angle > 1e-4f) {
+        // Rodrigues' formula R = cI + s[k]x + (1 - c) k k^T for the unit axis k.
+        const float nn = sqrtf(sum3(rx * rx, ry * ry, rz * rz));
+        const float kx = rx / nn, ky = ry / nn, kz = rz / nn;
+        const float s = sinf(angle), co = cosf(angle), oc = 1.0f - co;
+        Rd[0][0] = oc * kx * kx + co;
+        Rd[1][1] = oc * ky * ky + co;
+        Rd[2][2] = oc * kz * kz + co;
+        Rd[0][1] = oc * kx * ky - s * kz;
+        Rd[1][0] = oc * kx * ky + s * kz;
+        Rd[0][2] = oc * kx * kz + s * ky;
+        Rd[2][0] = oc * kx * kz - s * ky;
+        Rd[1][2] = oc * ky * kz - s * kx;
+        Rd[2][1] = oc * ky * kz + s * kx;
+    } else {
+        Rd[0][1] = -rz;
+        Rd[0][2] = ry;
+        Rd[1][0] = rz;
+        Rd[1][2] = -rx;
+        Rd[2][0] = -ry;
+        Rd[2][1] = rx;
+    }
+    float P[3][3], t[3];
+    for (int i = 0; i < 3; ++i) {
+        for (int j = 0; j < 3; ++j) P[i][j] = st->pose[j * 4 + i];
+        t[i] = st->pose[12 + i];
+    }
+    double Md[9];
+    float nt[3];
+    for (int i = 0; i < 3; ++i) {
+        for (int j = 0; j < 3; ++j) Md[i * 3 + j] = (double)((P[i][0] * Rd[0][j] + P[i][1] * Rd[1][j]) + P[i][2] * Rd[2][j]);
+        nt[i] = ((P[i][0] * x[0] + P[i][1] * x[1]) + P[i][2] * x[2]) + t[i];
+    }
+    double Rp[9];
+    projectSO3(Md, Rp);
+    for (int i = 0; i < 3; ++i) {
+        for (int j = 0; j < 3; ++j) st->pose[j * 4 + i] = (float)Rp[i * 3 + j];
+        st->pose[12 + i] = nt[i];
+    }
+    st->final_step = step_norm;
+    st->last_final_step = step_norm;
+    st->error = (float)(residual / (double)(inliers > 1 ? inliers : 1));
+    st->inliers = inliers;
+    if (step_norm < 5e-5f) {
+        st->converged = 1;
+        st->stop_level = level;
+    }
+}
+
+__global__ void icpInit(IcpState* st, PoseArg est, PoseArg ref) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    for (int i = 0; i < 16; ++i) {
+        st->pose[i] = poseEl(est, i);
+        st->ref[i] = poseEl(ref, i);
+    }
+    for (int i = 0; i < 36; ++i) st->information[i] = 0.0f;
+    st->error = 0.0f;
+    st->final_step = __builtin_inff();
+    st->last_final_step = __builtin_inff();
+    st->inliers = 0;
+    st->converged = 0;
+    for (int i = 0; i < 5; ++i) st->counters[i] = 0;
+    st->stop_level = -1;
+    st->iterations = 0;
+}
+
+struct PoseDerived {
+    M33 R_cw, R_rel;
+    F3 t_cw, t_rel;
+};
+
+__global__ __launch_bounds__(256) void icpIter(IcpIterArgs a, IcpState* __restrict__ st,
+                                               double* __restrict__ partials, unsigned* __restrict__ ticket) {
+    __shared__ PoseDerived sp;
+    __shared__ float s_hi[4][kIcpSlotsF], s_lo[4][kIcpSlotsF];
+    __shared__ int s_cnt[4][kIcpSlotsI];
+    __shared__ double s_sum[kIcpSlots];
+    __shared__ bool s_last;
+    if (st->stop_level == a.level) return;  // uniform over the grid
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    if (tid == 0) {
+        const Rigid pose = rigidFromPtr(st->pose);
+        const Rigid ref = rigidFromPtr(st->ref);
+        const Rigid rinv = rigidInverse(ref);
+        sp.R_cw = pose.R;
+        sp.t_cw = pose.t;
+        sp.R_rel = mulm(rinv.R, pose.R);
+        sp.t_rel = f3add(mulv(rinv.R, pose.t), rinv.t);
+    }
+    __syncthreads();
+    const M33 R_cw = sp.R_cw, R_rel = sp.R_rel;
+    const F3 t_cw = sp.t_cw, t_rel = sp.t_rel;
+
+    float hi[kIcpSlotsF], lo[kIcpSlotsF];
+#pragma unroll
+    for (int i = 0; i < kIcpSlotsF; ++i) {
+        hi[i] = 0.0f;
+        lo[i] = 0.0f;
+    }
+    int cnt[kIcpSlotsI] = {0, 0, 0, 0, 0, 0};
+    const int iw = a.W - 2, ih = a.H - 2;
+    const int npx = (iw > 0 && ih > 0) ? iw * ih : 0;
+    for (int p = blockIdx.x * 256 + tid; p < npx; p += gridDim.x * 256) {
+        const int x = 1 + p % iw, y = 1 + p / iw;
+        const int idx = y * a.W + x;
+        const F3 lv{a.lv[3 * idx], a.lv[3 * idx + 1], a.lv[3 * idx + 2]};
+        if (lv.z <= 0.001f || !f3finite(lv)) continue;
+        const F3 vr = f3add(mulv(R_rel, lv), t_rel);
+        if (vr.z <= 0.001f) continue;
+        cnt[1]++;
+        const float inv_z = 1.0f / vr.z;
+        const float mxf = a.Km.fx * vr.x * inv_z + a.Km.cx + 0.5f;
+        const float myf = a.Km.fy * vr.y * inv_z + a.Km.cy + 0.5f;
+        // floorToInt: finite and representable as int, then the image bounds.
+        if (!isfinite(mxf) || !isfinite(myf)) continue;
+        const float fmx = floorf(mxf), fmy = floorf(myf);
+        if (!(fmx >= 0.0f && fmx < (float)a.MW && fmy >= 0.0f && fmy < (float)a.MH)) continue;
+        cnt[3]++;
+        const int midx = (int)fmy * a.MW + (int)fmx;
+        const F3 mv{a.mv[3 * midx], a.mv[3 * midx + 1], a.mv[3 * midx + 2]};
+        const F3 mn{a.mn[3 * midx], a.mn[3 * midx + 1], a.mn[3 * midx + 2]};
+        if (!(f3finite(mv) && f3dot(mv, mv) > 1e-12f) || !(f3finite(mn) && f3dot(mn, mn) > 0.9f)) continue;
+        cnt[2]++;
+        const F3 vlw = f3add(mulv(R_cw, lv), t_cw);
+        const F3 dd = f3sub(vlw, mv);
+        if (f3dot(dd, dd) > a.dist_sq) {
+            cnt[4]++;
+            continue;
+        }
+        const F3 ln{a.ln[3 * idx], a.ln[3 * idx + 1], a.ln[3 * idx + 2]};
+        if (!(f3finite(ln) && f3dot(ln, ln) > 0.9f)) continue;
+        const F3 nlw = mulv(R_cw, ln);
+        if (fabsf(f3dot(nlw, mn)) < a.angle_cos) {
+            cnt[5]++;
+            continue;
+        }
+        const float err = f3dot(mn, dd);
+        if (!isfinite(err)) continue;
+        const F3 nlc = mulvT(R_cw, mn);
+        const F3 cr = f3cross(lv, nlc);
+        const float J[6] = {nlc.x, nlc.y, nlc.z, cr.x, cr.y, cr.z};
+        if (!(isfinite(J[0]) && isfinite(J[1]) && isfinite(J[2]) && isfinite(J[3]) && isfinite(J[4]) && isfinite(J[5])))
+            continue;
+        const float ae = fabsf(err);
+        const float w = (ae <= 0.02f) ? 1.0f : 0.02f / ae;
+        const float we = err * w;
+        const float loss = (ae <= 0.02f) ? ae * ae : 2.0f * 0.02f * ae - 0.02f * 0.02f;
+        int k = 0;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+#pragma unroll
+            for (int j = i; j < 6; ++j) {
+                twoSum(hi[k], lo[k], w * J[i] * J[j], 0.0f);
+                ++k;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 6; ++i) twoSum(hi[21 + i], lo[21 + i], -(J[i] * we), 0.0f);
+        twoSum(hi[27], lo[27], loss, 0.0f);
+        cnt[0]++;
+    }
+    // Wave fold (compensated pairs, exact integers).
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+        for (int i = 0; i < kIcpSlotsF; ++i) {
+            const float oh = __shfl_xor(hi[i], o), ol = __shfl_xor(lo[i], o);
+            twoSum(hi[i], lo[i], oh, ol);
+        }
+#pragma unroll
+        for (int i = 0; i < kIcpSlotsI; ++i) cnt[i] += __shfl_xor(cnt[i], o);
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int i = 0; i < kIcpSlotsF; ++i) {
+            s_hi[wv][i] = hi[i];
+            s_lo[wv][i] = lo[i];
+        }
+#pragma unroll
+        for (int i = 0; i < kIcpSlotsI; ++i) s_cnt[wv][i] = cnt[i];
+    }
+    __syncthreads();
+    const unsigned nb = gridDim.x;
+    if (tid < kIcpSlots) {
+        double v;
+        if (tid < kIcpSlotsF) {
+            float h = s_hi[0][tid], l = s_lo[0][tid];
+            for (int w2 = 1; w2 < 4; ++w2) twoSum(h, l, s_hi[w2][tid], s_lo[w2][tid]);
+            v = (double)h + (double)l;
+        } else {
+            const int i = tid - kIcpSlotsF;
+            v = (double)(s_cnt[0][i] + s_cnt[1][i] + s_cnt[2][i] + s_cnt[3][i]);
+        }
+        partials[(size_t)tid * nb + blockIdx.x] = v;
+    }
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) {
+        const unsigned prev = atomicInc(ticket, nb - 1);  // wraps to 0 for the next launch
+        s_last = (prev == nb - 1);
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    // Last block: FP64 cross-block sum in block order, one wave per slot group.
+    for (int slot = wv; slot < kIcpSlots; slot += 4) {
+        double s = 0.0;
+        for (unsigned bi = lane; bi < nb; bi += 64) s += partials[(size_t)slot * nb + bi];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+        if (lane == 0) s_sum[slot] = s;
+    }
+    __syncthreads();
+    if (tid == 0) icpSolve(st, s_sum, a.level, a.iter);
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers.
+// ---------------------------------------------------------------------------
+size_t icpStateBytes() { return sizeof(IcpState); }
+int icpBlocksFor(int W, int H) {
+    const long long npx = (long long)(W > 2 ? W - 2 : 0) * (H > 2 ? H - 2 : 0);
+    long long b = (npx + 511) / 512;
+    if (b < 1) b = 1;
+    if (b > 1024) b = 1024;
+    return (int)b;
+}
+size_t icpPartialsBytes(int maxBlocks) { return sizeof(double) * kIcpSlots * (size_t)maxBlocks; }
+
+hipError_t launchIcpInit(void* st, const PoseArg& est, const PoseArg& ref, hipStream_t s) {
+    hipLaunchKernelGGL(icpInit, dim3(1), dim3(64), 0, s, (IcpState*)st, est, ref);
+    return hipGetLastError();
+}
+
+hipError_t launchIcpIter(const float* lv, const float* ln, int W, int H, const float* mv, const float* mn, int MW,
+                         int MH, Intr Km, float angle_cos, float dist_sq, int level, int iter, void* st,
+                         double* partials, unsigned* ticket, hipStream_t s) {
+    IcpIterArgs a;
+    a.lv = lv;
+    a.ln = ln;
+    a.W = W;
+    a.H = H;
+    a.mv = mv;
+    a.mn = mn;
+    a.MW = MW;
+    a.MH = MH;
+    a.Km = Km;
+    a.angle_cos = angle_cos;
+    a.dist_sq = dist_sq;
+    a.level = level;
+    a.iter = iter;
+    hipLaunchKernelGGL(icpIter, dim3(icpBlocksFor(W, H)), dim3(256), 0, s, a, (IcpState*)st, partials, ticket);
+    return hipGetLastError();
+}
+
+// Host view of the state (copied back once per frame).
+void icpStateRead(const void* hostCopy, float* pose, float* information, float* error, float* final_step,
+                  int* inliers, int* converged, int* counters, int* iterations) {
+    const IcpState* s = static_cast<const IcpState*>(hostCopy);
+    for (int i = 0; i < 16; ++i) pose[i] = s->pose[i];
+    for (int i = 0; i < 36; ++i) information[i] = s->information[i];
+    *error = s->error;
+    *final_step = s->last_final_step;
+    *inliers = s->inliers;
+    *converged = s->converged;
+    for (int i = 0; i < 5; ++i) counters[i] = s->counters[i];
+    *iterations = s->iterations;
+}
+
+}  // namespace azu

@@ -1,0 +1,356 @@
+// TSDF integration on gfx950: voxel-projective update over the camera frustum's
+// bounding box (the reference's integrateCPU, src/tsdf/TSDFVolume.cpp:214-346).
+//
+// Two kernels:
+//
+//  * integrateExact<Store>: the FP32 parity path. The reference carries each row's
+//    camera coordinate by an incremental x-step (c += R_wc.col(0) * vs,
+//    TSDFVolume.cpp:286-287), so voxel x's coordinate is the result of (x - x0)
+//    sequential float adds. To reproduce it bit for bit while keeping HBM access
+//    coalesced, every wave owns 64 rows (consecutive z at one y): in phase 1 lane r
+//    walks row r through a 32-column chunk, runs the projection, depth gather and the
+//    gates, and leaves tsdf_new (or NaN = skip) in LDS; in phase 2 the wave reads the
+//    chunk transposed, lane = column, so each read-modify-write covers 32 consecutive
+//    voxels of two rows. Division and square root are IEEE-exact, nothing contracts.
+//
+//  * integrateFast<Store>: the FP16 + UInt16 storage form. One thread owns 4
+//    consecutive voxels of a row (one 16-byte word for FP16/U16), the camera
+//    coordinate is an affine function of x evaluated with fmaf, reciprocals are the
+//    hardware v_rcp_f32. Accuracy is stated against the FP64 oracle.
+//
+// Both read the pose by value or from device memory (the ICP output), so a frame can
+// run with no host synchronisation. Updated / newly observed voxel counts are
+// accumulated per wave (ballot popcount) into a 2-entry device counter when given.
+#include <algorithm>
+
+#include "common.hpp"
+
+namespace azu {
+
+struct IntFrame {
+    M33 R_wc;
+    F3 t_wc;
+    M33 R_cw;
+    F3 t_cw;
+    F3 step;
+    int x0, x1, y0, y1, z0, z1;
+    bool ok;
+};
+
+// Frustum AABB and camera transform exactly as the CPU path computes them
+// (TSDFVolume.cpp:238-266).
+__device__ __forceinline__ IntFrame integrateSetup(const VolParams& p, const PoseArg& pa, Intr K, int W, int H,
+                                                   float max_d) {
+    IntFrame f;
+    const Rigid pose = rigidFrom(pa);
+    const Rigid w2c = rigidInverse(pose);
+    f.R_wc = w2c.R;
+    f.t_wc = w2c.t;
+    f.R_cw = pose.R;
+    f.t_cw = pose.t;
+    const float vs = p.vs;
+    const F3 origin{p.origin[0], p.origin[1], p.origin[2]};
+    F3 lo = f3div(f3sub(pose.t, origin), vs);
+    F3 hi = lo;
+    const float z_far = max_d + p.trunc;
+    const float cpx[4][2] = {{0.0f, 0.0f}, {(float)W, 0.0f}, {0.0f, (float)H}, {(float)W, (float)H}};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const F3 ray{(cpx[i][0] - K.cx) / K.fx * z_far, (cpx[i][1] - K.cy) / K.fy * z_far, z_far};
+        const F3 v = f3div(f3sub(f3add(mulv(pose.R, ray), pose.t), origin), vs);
+        lo = {smin(lo.x, v.x), smin(lo.y, v.y), smin(lo.z, v.z)};
+        hi = {smax(hi.x, v.x), smax(hi.y, v.y), smax(hi.z, v.z)};
+    }
+    f.ok = f3finite(lo) && f3finite(hi);
+    const float top = (float)(p.res - 1);
+    auto clampIdx = [top](float v) { return (int)smax(0.0f, smin(top, v)); };
+    f.x0 = clampIdx(floorf(lo.x));
+    f.x1 = clampIdx(ceilf(hi.x));
+    f.y0 = clampIdx(floorf(lo.y));
+    f.y1 = clampIdx(ceilf(hi.y));
+    f.z0 = clampIdx(floorf(lo.z));
+    f.z1 = clampIdx(ceilf(hi.z));
+    f.step = {f.R_wc.m[0][0] * vs, f.R_wc.m[1][0] * vs, f.R_wc.m[2][0] * vs};
+    return f;
+}
+
+__device__ __forceinline__ void countWave(unsigned long long* counters, bool upd, bool newly) {
+    if (!counters) return;
+    const unsigned long long bu = __ballot(upd), bn = __ballot(newly);
+    if ((threadIdx.x & 63) == 0 && (bu | bn)) {
+        if (bu) atomicAdd(&counters[0], (unsigned long long)__popcll(bu));
+        if (bn) atomicAdd(&counters[1], (unsigned long long)__popcll(bn));
+    }
+}
+
+constexpr int kChunk = 32;          // x columns per phase
+constexpr int kLdsStride = kChunk + 1;
+
+template <class S>
+__global__ __launch_bounds__(256) void integrateExact(typename S::Word* __restrict__ vox, VolParams p, Intr K,
+                                                      int W, int H, const float* __restrict__ depth, float min_d,
+                                                      float max_d, PoseArg pa,
+                                                      unsigned long long* __restrict__ counters) {
+    __shared__ float lds[4][64 * kLdsStride];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const IntFrame f = integrateSetup(p, pa, K, W, H, max_d);
+    if (!f.ok) return;
+    const int ny = f.y1 - f.y0 + 1;
+    const int nzg = (f.z1 - f.z0 + 1 + 63) / 64;
+    const int task = blockIdx.x * 4 + wv;
+    if (task >= ny * nzg) return;  // whole wave; the kernel uses no block barrier
+    const int y = f.y0 + task % ny;
+    const int zb = f.z0 + (task / ny) * 64;
+    const int z = zb + lane;
+    const bool row_ok = z <= f.z1;
+    const float vs = p.vs, trunc = p.trunc;
+    const F3 pp{p.origin[0] + (float)f.x0 * vs, p.origin[1] + (float)y * vs, p.origin[2] + (float)z * vs};
+    F3 c = f3add(mulv(f.R_wc, pp), f.t_wc);
+    float* L = lds[wv];
+    const size_t res = (size_t)p.res, plane = res * res;
+    const float fW = (float)W, fH = (float)H;
+
+    for (int xc = f.x0; xc <= f.x1; xc += kChunk) {
+        const int n = min(kChunk, f.x1 - xc + 1);
+        // Phase 1: lane = row, serial over the chunk's columns.
+#pragma unroll 4
+        for (int j = 0; j < kChunk; ++j) {
+            float tn = __builtin_nanf("");
+            if (j < n) {
+                if (row_ok && c.z > 0.0f) {
+                    const float inv_z = 1.0f / c.z;
+                    const float uf = floorf(K.fx * c.x * inv_z + K.cx + 0.5f);
+                    const float vf = floorf(K.fy * c.y * inv_z + K.cy + 0.5f);
+                    if (uf >= 0.0f && uf < fW && vf >= 0.0f && vf < fH) {
+                        const float d = depth[(int)vf * W + (int)uf];
+                        if (d >= min_d && d <= max_d) {
+                            const float sdf = d - c.z;
+                            if (!(sdf < -trunc)) tn = smin(1.0f, sdf / trunc);
+                        }
+                    }
+                }
+                c = f3add(c, f.step);
+            }
+            L[lane * kLdsStride + j] = tn;
+        }
+        waveSync();
+        // Phase 2: lane = column, two rows per instruction.
+        const int j = lane & (kChunk - 1), g = lane >> 5;
+        const bool col_ok = j < n;
+#pragma unroll 8
+        for (int k = 0; k < 32; ++k) {
+            const int r = g + 2 * k;
+            const int zz = zb + r;
+            const float tn = L[r * kLdsStride + j];
+            const bool upd = col_ok && zz <= f.z1 && !__builtin_isnan(tn);
+            bool newly = false;
+            if (upd) {
+                typename S::Word* ptr = vox + ((size_t)zz * plane + (size_t)y * res + (size_t)(xc + j));
+                const typename S::Word wd = *ptr;
+                const float w_old = S::weight(wd);
+                const float t_old = S::tsdf(wd);
+                newly = (w_old == 0.0f);
+                *ptr = S::pack((t_old * w_old + tn) / (w_old + 1.0f), smin(w_old + 1.0f, p.max_w));
+            }
+            countWave(counters, upd, newly);
+        }
+        waveSync();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fast kernel. Block = 256 threads over a tile of 64 x-voxels (16 quads) x 16 rows (y)
+// at one z; grid covers the whole volume, tiles outside the AABB exit at once.
+// ---------------------------------------------------------------------------
+template <class S> struct QuadIO;
+template <> struct QuadIO<StoreF16> {
+    using Q = uint4;
+    __device__ static Q load(const uint32_t* p) { return *reinterpret_cast<const uint4*>(p); }
+    __device__ static void store(uint32_t* p, Q q) { *reinterpret_cast<uint4*>(p) = q; }
+    __device__ static uint32_t get(const Q& q, int i) { return i == 0 ? q.x : i == 1 ? q.y : i == 2 ? q.z : q.w; }
+    __device__ static void set(Q& q, int i, uint32_t v) {
+        if (i == 0) q.x = v; else if (i == 1) q.y = v; else if (i == 2) q.z = v; else q.w = v;
+    }
+};
+template <> struct QuadIO<StoreF32> {
+    struct Q {
+        float4 a, b;
+    };
+    __device__ static Q load(const float2* p) {
+        const float4* q = reinterpret_cast<const float4*>(p);
+        return Q{q[0], q[1]};
+    }
+    __device__ static void store(float2* p, const Q& v) {
+        float4* q = reinterpret_cast<float4*>(p);
+        q[0] = v.a;
+        q[1] = v.b;
+    }
+    __device__ static float2 get(const Q& q, int i) {
+        return i == 0 ? make_float2(q.a.x, q.a.y) : i == 1 ? make_float2(q.a.z, q.a.w)
+             : i == 2 ? make_float2(q.b.x, q.b.y) : make_float2(q.b.z, q.b.w);
+    }
+    __device__ static void set(Q& q, int i, float2 v) {
+        if (i == 0) { q.a.x = v.x; q.a.y = v.y; }
+        else if (i == 1) { q.a.z = v.x; q.a.w = v.y; }
+        else if (i == 2) { q.b.x = v.x; q.b.y = v.y; }
+        else { q.b.z = v.x; q.b.w = v.y; }
+    }
+};
+
+template <class S>
+__global__ __launch_bounds__(256) void integrateFast(typename S::Word* __restrict__ vox, VolParams p, Intr K, int W,
+                                                     int H, const float* __restrict__ depth, float min_d, float max_d,
+                                                     PoseArg pa, unsigned long long* __restrict__ counters) {
+    const IntFrame f = integrateSetup(p, pa, K, W, H, max_d);
+    if (!f.ok) return;
+    const int tx0 = blockIdx.x * 64, ty0 = blockIdx.y * 16, z = blockIdx.z;
+    if (tx0 > f.x1 || tx0 + 63 < f.x0 || ty0 > f.y1 || ty0 + 15 < f.y0 || z < f.z0 || z > f.z1) return;
+    const int q = threadIdx.x & 15;
+    const int y = ty0 + (threadIdx.x >> 4);
+    const int xq = tx0 + q * 4;
+    const bool row_ok = y >= f.y0 && y <= f.y1 && y < p.res;
+    const float vs = p.vs;
+    // Camera coordinate of voxel x: c(x) = base + x * ax, with the projection numerators
+    // u_num = fx c.x + (cx + 1/2) c.z and v_num = fy c.y + (cy + 1/2) c.z affine in x too.
+    const F3 pw{p.origin[0], p.origin[1] + (float)y * vs, p.origin[2] + (float)z * vs};
+    const F3 base = f3add(mulv(f.R_wc, pw), f.t_wc);
+    const F3 ax = f.step;
+    const float cu = K.cx + 0.5f, cv = K.cy + 0.5f;
+    const float bu = fmaf(K.fx, base.x, cu * base.z), bv = fmaf(K.fy, base.y, cv * base.z);
+    const float su = fmaf(K.fx, ax.x, cu * ax.z), sv = fmaf(K.fy, ax.y, cv * ax.z);
+    const float inv_trunc = 1.0f / p.trunc, neg_trunc = -p.trunc;
+    float tn[4];
+    bool any = false;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        tn[i] = __builtin_nanf("");
+        const int x = xq + i;
+        if (!row_ok || x < f.x0 || x > f.x1) continue;
+        const float fx = (float)x;
+        const float cz = fmaf(fx, ax.z, base.z);
+        if (!(cz > 0.0f)) continue;
+        const float rz = __builtin_amdgcn_rcpf(cz);
+        const float uf = floorf(fmaf(fx, su, bu) * rz);
+        const float vf = floorf(fmaf(fx, sv, bv) * rz);
+        if (!(uf >= 0.0f && uf < (float)W && vf >= 0.0f && vf < (float)H)) continue;
+        const float d = depth[(int)vf * W + (int)uf];
+        if (!(d >= min_d && d <= max_d)) continue;
+        const float sdf = d - cz;
+        if (sdf < neg_trunc) continue;
+        tn[i] = fminf(1.0f, sdf * inv_trunc);
+        any = true;
+    }
+    bool upd_any = false;
+    int nupd = 0, nnew = 0;
+    if (any) {
+        typename S::Word* ptr = vox + ((size_t)z * p.res + (size_t)y) * (size_t)p.res + (size_t)xq;
+        typename QuadIO<S>::Q qv = QuadIO<S>::load(ptr);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            if (__builtin_isnan(tn[i])) continue;
+            const typename S::Word wd = QuadIO<S>::get(qv, i);
+            const float w_old = S::weight(wd);
+            const float t_old = S::tsdf(wd);
+            const float t_new = fmaf(t_old, w_old, tn[i]) * __builtin_amdgcn_rcpf(w_old + 1.0f);
+            QuadIO<S>::set(qv, i, S::pack(t_new, fminf(w_old + 1.0f, p.max_w)));
+            ++nupd;
+            nnew += (w_old == 0.0f);
+        }
+        QuadIO<S>::store(ptr, qv);
+        upd_any = true;
+    }
+    if (counters) {
+        // Per-wave sums of the per-thread counts.
+        int a = nupd, b = nnew;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            a += __shfl_xor(a, o);
+            b += __shfl_xor(b, o);
+        }
+        const bool anyw = __ballot(upd_any) != 0;
+        if ((threadIdx.x & 63) == 0 && anyw) {
+            atomicAdd(&counters[0], (unsigned long long)a);
+            if (b) atomicAdd(&counters[1], (unsigned long long)b);
+        }
+    }
+}
+
+// Reset to the empty state.
+template <class S>
+__global__ void fillEmpty(typename S::Word* __restrict__ vox, size_t n) {
+    const typename S::Word e = S::empty();
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        vox[i] = e;
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers (called from capi.hip).
+// ---------------------------------------------------------------------------
+hipError_t launchIntegrate(int format, bool exact, void* vox, const VolParams& p, Intr K, int W, int H,
+                           const float* depth, float min_d, float max_d, const PoseArg& pa,
+                           unsigned long long* counters, hipStream_t st) {
+    if (exact) {
+        // One wave per (y, 64-row z group) of the whole volume; waves outside the
+        // frustum box exit at once (the box is only known on the device).
+        const long long tasks = (long long)p.res * ((p.res + 63) / 64);
+        const unsigned blocks = (unsigned)((tasks + 3) / 4);
+        if (format == kFmtF32)
+            hipLaunchKernelGGL(integrateExact<StoreF32>, dim3(blocks), dim3(256), 0, st, (float2*)vox, p, K, W, H,
+                               depth, min_d, max_d, pa, counters);
+        else
+            hipLaunchKernelGGL(integrateExact<StoreF16>, dim3(blocks), dim3(256), 0, st, (uint32_t*)vox, p, K, W,
+                               H, depth, min_d, max_d, pa, counters);
+    } else {
+        const dim3 grid((p.res + 63) / 64, (p.res + 15) / 16, p.res);
+        if (format == kFmtF32)
+            hipLaunchKernelGGL(integrateFast<StoreF32>, grid, dim3(256), 0, st, (float2*)vox, p, K, W, H, depth,
+                               min_d, max_d, pa, counters);
+        else
+            hipLaunchKernelGGL(integrateFast<StoreF16>, grid, dim3(256), 0, st, (uint32_t*)vox, p, K, W, H, depth,
+                               min_d, max_d, pa, counters);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launchFillEmpty(int format, void* vox, size_t n, hipStream_t st) {
+    const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 65536);
+    if (format == kFmtF32)
+        hipLaunchKernelGGL(fillEmpty<StoreF32>, dim3(blocks), dim3(256), 0, st, (float2*)vox, n);
+    else
+        hipLaunchKernelGGL(fillEmpty<StoreF16>, dim3(blocks), dim3(256), 0, st, (uint32_t*)vox, n);
+    return hipGetLastError();
+}
+
+}  // namespace azu
+
+namespace azu {
+
+// Whole-volume conversion between the storage words and float tsdf / weight arrays
+// (download / upload).
+template <class S>
+__global__ void volConvert(typename S::Word* __restrict__ vox, float* __restrict__ tsdf, float* __restrict__ weight,
+                           size_t n, int to_float) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        if (to_float) {
+            const typename S::Word w = vox[i];
+            tsdf[i] = S::tsdf(w);
+            weight[i] = S::weight(w);
+        } else {
+            vox[i] = S::pack(tsdf[i], weight[i]);
+        }
+    }
+}
+
+hipError_t launchVolConvert(int format, bool to_float, void* vox, float* tsdf, float* weight, size_t n,
+                            hipStream_t st) {
+    const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 65536);
+    if (format == kFmtF32)
+        hipLaunchKernelGGL(volConvert<StoreF32>, dim3(blocks), dim3(256), 0, st, (float2*)vox, tsdf, weight, n,
+                           to_float ? 1 : 0);
+    else
+        hipLaunchKernelGGL(volConvert<StoreF16>, dim3(blocks), dim3(256), 0, st, (uint32_t*)vox, tsdf, weight, n,
+                           to_float ? 1 : 0);
+    return hipGetLastError();
+}
+
+}  // namespace azu

@@ -1,0 +1,187 @@
+/*
+ * azu_hip.h - C ABI of the MI355X (gfx950) KinectFusion frame loop.
+ *
+ * This is the drop-in boundary below the host classes include/tsdf/TSDFVolume.h and
+ * include/tracking/ICPTracker.h. Plain pointers and sizes only; every call returns an
+ * azu_status and never throws; azu_last_error() gives the message of the calling
+ * thread's last failure. Pointers named d_* are device pointers (azu_malloc or any HIP
+ * allocation on the current device); everything else is host memory. Calls on one
+ * handle must be serialised by the caller; work is enqueued on the given stream
+ * (NULL = the default stream) and is asynchronous unless the comment says it
+ * synchronises.
+ *
+ * Conventions shared with the reference (iphands/azu):
+ *   - poses are 4x4 column-major float[16] (Eigen's default storage), world-from-camera;
+ *   - intrinsics K = {fx, fy, cx, cy} in pixels;
+ *   - vectors (vertices, normals) are packed xyz float triples (Eigen::Vector3f / float3);
+ *   - a volume index is z*res*res + y*res + x, voxel corner at origin + (x,y,z)*voxel_size;
+ *   - the empty voxel is tsdf = 1, weight = 0.
+ */
+#ifndef AZU_HIP_H
+#define AZU_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AZU_HIP_ABI_VERSION 1
+
+typedef enum azu_status {
+    AZU_OK = 0,
+    AZU_ERR_ARG = 1,   /* invalid argument (null handle, bad size, unsupported format) */
+    AZU_ERR_HIP = 2,   /* a HIP runtime call or kernel launch failed */
+    AZU_ERR_ALLOC = 3, /* device or pinned allocation failed */
+    AZU_ERR_STATE = 4  /* call not valid in the object's current state */
+} azu_status;
+
+/* Voxel storage forms. F32: float tsdf + float weight (8 B/voxel). F16_U16: binary16
+ * tsdf + uint16 weight (4 B/voxel). The reference stores a 20 B AoS voxel with colour
+ * (include/tsdf/TSDFVolume.h:55-62); colour is not stored here. */
+typedef enum azu_format { AZU_FMT_F32 = 0, AZU_FMT_F16_U16 = 1 } azu_format;
+
+/* Kernel arithmetic. EXACT: the FP32 parity kernels (IEEE division, no contraction,
+ * the reference's incremental x-step); FAST: hardware reciprocals and FMA. AUTO picks
+ * EXACT for F32 and FAST for F16_U16. */
+typedef enum azu_mode { AZU_MODE_AUTO = 0, AZU_MODE_EXACT = 1, AZU_MODE_FAST = 2 } azu_mode;
+
+typedef void* azu_stream; /* hipStream_t */
+
+/* Mirrors kfusion::tsdf::TSDFParams (include/tsdf/TSDFVolume.h:39-53). */
+typedef struct azu_tsdf_params {
+    int resolution;
+    float voxel_size;
+    float truncation;
+    float max_weight;
+    float origin[3];
+    float min_depth;
+    float max_depth;
+} azu_tsdf_params;
+
+/* Mirrors kfusion::tracking::ICPParams (include/tracking/ICPTracker.h:19-25). */
+typedef struct azu_icp_params {
+    int max_iterations[3]; /* per pyramid level 0, 1, 2 */
+    float dist_threshold;  /* m */
+    float angle_threshold; /* degrees, sanitised to [0, 85] (NaN -> 30) */
+    float min_depth;
+    float max_depth;
+} azu_icp_params;
+
+/* Mirrors kfusion::tracking::ICPResult (include/tracking/ICPTracker.h:27-47). */
+typedef struct azu_icp_result {
+    float pose[16];
+    float error;
+    int inliers;
+    int converged;
+    int tracking_ok;
+    float final_step;
+    float information[36]; /* row-major, undamped J^T W J of the last solve */
+    int valid_live_points;
+    int valid_model_points;
+    int projected_points;
+    int dist_filtered;
+    int angle_filtered;
+    int iterations; /* solves run over all levels (diagnostic) */
+} azu_icp_result;
+
+typedef struct azu_volume azu_volume;
+typedef struct azu_tracker azu_tracker;
+
+/* ---- library ---------------------------------------------------------------- */
+const char* azu_last_error(void);
+int azu_abi_version(void);
+/* Current device id, its name (name_len bytes incl. NUL) and HBM size. */
+int azu_device_info(int* device, char* name, size_t name_len, size_t* hbm_bytes);
+
+/* ---- device memory and streams (thin wrappers, for callers without HIP headers) ---- */
+int azu_malloc(void** d_ptr, size_t bytes);
+int azu_free(void* d_ptr);
+int azu_host_alloc(void** ptr, size_t bytes); /* pinned host memory */
+int azu_host_free(void* ptr);
+int azu_memcpy_h2d(void* d_dst, const void* src, size_t bytes, azu_stream s);
+int azu_memcpy_d2h(void* dst, const void* d_src, size_t bytes, azu_stream s);
+int azu_memcpy_d2d(void* d_dst, const void* d_src, size_t bytes, azu_stream s);
+int azu_stream_create(azu_stream* s);
+int azu_stream_destroy(azu_stream s);
+int azu_stream_sync(azu_stream s); /* synchronises */
+
+/* ---- TSDF volume (kfusion::tsdf::TSDFVolume, include/tsdf/TSDFVolume.h:69-190) ---- */
+/* Allocates res^3 voxels in the given format and resets them to the empty state
+ * (TSDFVolume::TSDFVolume, src/tsdf/TSDFVolume.cpp:96-102). Synchronises. */
+int azu_vol_create(const azu_tsdf_params* p, int format, azu_volume** out);
+int azu_vol_destroy(azu_volume* v);
+/* TSDFVolume::reset (TSDFVolume.cpp:112-122). */
+int azu_vol_reset(azu_volume* v, azu_stream s);
+/* TSDFVolume::setParams (TSDFVolume.cpp:124-146): any field difference clears the
+ * volume; a resolution change reallocates. Synchronises. */
+int azu_vol_set_params(azu_volume* v, const azu_tsdf_params* p);
+int azu_vol_get_params(const azu_volume* v, azu_tsdf_params* p, int* format);
+int azu_vol_set_mode(azu_volume* v, int mode);
+/* Whole-volume copies to / from host float arrays of res^3 (voxelData / voxelAt).
+ * Synchronise. */
+int azu_vol_download(azu_volume* v, float* tsdf, float* weight);
+int azu_vol_upload(azu_volume* v, const float* tsdf, const float* weight);
+/* Raw device storage (format-dependent words, index as above). */
+int azu_vol_storage(azu_volume* v, void** d_ptr, size_t* bytes);
+/* Voxels updated / newly observed by integrate since the last reset of the counters
+ * (reset != 0 clears them after reading). Synchronises. */
+int azu_vol_counters(azu_volume* v, unsigned long long* updated, unsigned long long* newly, int reset);
+int azu_vol_integrated_frames(const azu_volume* v, int* frames);
+
+/* TSDFVolume::integrate / integrateGPU (TSDFVolume.cpp:148-346, TSDFVolume.h:89-96,
+ * 169-176): d_depth_m is W*H float metres on the device. */
+int azu_integrate(azu_volume* v, const float* d_depth_m, int W, int H, const float K[4], const float pose[16],
+                  float min_depth, float max_depth, azu_stream s);
+/* Same, pose read from device memory (16 floats) when the kernel runs. */
+int azu_integrate_dpose(azu_volume* v, const float* d_depth_m, int W, int H, const float K[4], const float* d_pose,
+                        float min_depth, float max_depth, azu_stream s);
+
+/* TSDFVolume::raycast / raycastGPU (TSDFVolume.cpp:348-476, TSDFVolume.h:99-104,
+ * 177-181): W*H packed xyz vertices and normals on the device; a miss is all zeros.
+ * d_t_hit (optional, may be NULL) receives the ray parameter of each hit (NaN on a miss). */
+int azu_raycast(azu_volume* v, const float pose[16], const float K[4], int W, int H, float* d_vertices,
+                float* d_normals, azu_stream s);
+int azu_raycast_dpose(azu_volume* v, const float* d_pose, const float K[4], int W, int H, float* d_vertices,
+                      float* d_normals, float* d_t_hit, azu_stream s);
+
+/* ---- ICP tracker (kfusion::tracking::ICPTracker, include/tracking/ICPTracker.h:87-134) ---- */
+/* W x H is the live frame size; K its intrinsics (also the camera of the model image). */
+int azu_tracker_create(int W, int H, const float K[4], const azu_icp_params* p, azu_tracker** out);
+int azu_tracker_destroy(azu_tracker* t);
+int azu_tracker_set_params(azu_tracker* t, const azu_icp_params* p);
+/* ICPTracker::prepareLiveGPU (ICPTracker.h:108) with the CPU semantics of buildFrameData
+ * + computeNormals + buildFramePyramid (src/sensor/FrameData.cpp:113-345): depth in
+ * uint16 millimetres (x 0.001 m) or float metres, band-gated to [min_depth, max_depth]. */
+int azu_tracker_prepare_mm(azu_tracker* t, const uint16_t* d_depth_mm, azu_stream s);
+int azu_tracker_prepare_m(azu_tracker* t, const float* d_depth_m, azu_stream s);
+/* Device buffers of pyramid level 0..2 (depth metres, packed vertices / normals). */
+int azu_tracker_level(azu_tracker* t, int level, const float** d_depth, const float** d_vertices,
+                      const float** d_normals, int* W, int* H);
+/* ICPTracker::track / trackPreparedGPU (ICPTracker.cpp:40-134) against a model image
+ * (MW x MH packed world-space vertices / normals on the device, raycast at ref_pose).
+ * Enqueues every iteration; read the outcome with azu_tracker_result. */
+int azu_tracker_track(azu_tracker* t, const float* d_model_v, const float* d_model_n, int MW, int MH,
+                      const float pose_estimate[16], const float ref_pose[16], azu_stream s);
+/* Same, estimate and reference pose read from device memory when the first kernel runs. */
+int azu_tracker_track_dpose(azu_tracker* t, const float* d_model_v, const float* d_model_n, int MW, int MH,
+                            const float* d_pose_estimate, const float* d_ref_pose, azu_stream s);
+/* Device pointer to the tracked pose (16 floats), valid once the enqueued track ran. */
+int azu_tracker_pose_device(azu_tracker* t, const float** d_pose);
+/* Synchronises the stream and returns the result of the last track. */
+int azu_tracker_result(azu_tracker* t, azu_icp_result* out, azu_stream s);
+
+/* ---- per-kernel timing (HIP events on the launch stream) ---- */
+/* While enabled, every kernel launch is bracketed by events; azu_profile_read
+ * synchronises and returns, per kernel name, launches and total milliseconds.
+ * names: n_max entries of 48 chars. Returns the number of entries in *n. */
+int azu_profile_enable(int on);
+int azu_profile_reset(void);
+int azu_profile_read(char* names, int n_max, long long* launches, double* total_ms, int* n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AZU_HIP_H */
