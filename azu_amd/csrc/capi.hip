@@ -615,6 +615,13 @@ int azu_tracker_track_dpose(azu_tracker* t, const float* mv, const float* mn, in
     return trackImpl(t, mv, mn, MW, MH, poseByPtr(d_est), poseByPtr(d_ref), s);
 }
 
+int azu_tracker_set_pose(azu_tracker* t, const float pose[16], azu_stream s) {
+    if (!t || !pose) return fail(AZU_ERR_ARG, "null argument");
+    ProfScope ps("icp_init", (hipStream_t)s);
+    AZU_HIP_TRY(launchIcpInit(t->state, poseByValue(pose), poseByValue(pose), (hipStream_t)s), "pose set launch");
+    return AZU_OK;
+}
+
 int azu_tracker_pose_device(azu_tracker* t, const float** d_pose) {
     if (!t || !d_pose) return fail(AZU_ERR_ARG, "null argument");
     *d_pose = static_cast<const float*>(t->state);  // IcpState::pose is its first member

@@ -114,6 +114,7 @@ _SIGS = {
                                          c_float_p, c_void_p]),
     "azu_tracker_track_dpose": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_int, ctypes.c_int, c_void_p,
                                                c_void_p, c_void_p]),
+    "azu_tracker_set_pose": (ctypes.c_int, [c_void_p, c_float_p, c_void_p]),
     "azu_tracker_pose_device": (ctypes.c_int, [c_void_p, ctypes.POINTER(c_void_p)]),
     "azu_tracker_result": (ctypes.c_int, [c_void_p, ctypes.POINTER(IcpResult), c_void_p]),
     "azu_profile_enable": (ctypes.c_int, [ctypes.c_int]),

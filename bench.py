@@ -1,0 +1,208 @@
+"""Frame-loop benchmark: frames/s of integrate + raycast + 3-level ICP on one MI355X.
+
+Workload: a 512^3 TSDF at 2 mm over a 1.024 m cube, 640x480 synthetic uint16 depth
+frames (sphere r = 0.3 m before the plane z = 1.2 m, 10 seeded boxes, orbit 2 deg /
+frame with 5 mm / 0.5 deg jitter, Gaussian depth noise 0.0012 z^2), ICP {10, 5, 4}
+iterations. Each step is one frame of the loop in the reference pipeline's order
+(live pyramid -> ICP against the model raycast at the previous pose -> integrate ->
+raycast). Frames are uploaded to HBM before the timed region; the tracked pose stays
+on the device and is copied out asynchronously, so the timed region is GPU work only.
+
+`value` is the frames/s of the FP16 + UInt16 TSDF (4 B/voxel); the plain FP32 TSDF
+(8 B/voxel, the bit-exact parity kernels) runs the same frames and is reported beside
+it. `roofline` is for the dominant kernel of the FP16 form (the one with the most
+time per frame), from HIP events on its launch stream; `cpu_baseline` times the CPU
+oracle (FP32 restatement of the reference CPU path with the reference's 20 B AoS
+voxel) on the first frames of the same sequence.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "frames/s (integrate+raycast+ICP) at 512³, FP16 vs FP32 TSDF; % HBM peak"
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=97)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="c2_512")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-frames", type=int, default=8)
+    ap.add_argument("--profile-frames", type=int, default=24)
+    ap.add_argument("--only", choices=["both", "f16", "f32"], default="both")
+    return ap.parse_args()
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_form(frames_dev, nframes, pose0, params, fmt, W, H, K, warmup, steps):
+    from azu_amd import native as N
+    from azu_amd.pipeline import FrameLoop
+
+    loop = FrameLoop(params, fmt, W, H, K)
+    loop.reset(pose0)
+    for f in range(warmup):
+        loop.frame(frames_dev[f])
+    N.synchronize()
+    t0 = time.perf_counter()
+    for f in range(warmup, warmup + steps):
+        loop.frame(frames_dev[f])
+    N.synchronize()
+    dt = time.perf_counter() - t0
+    poses = loop.poses()
+    loop.close()
+    return dt, poses
+
+
+def profile_form(frames_dev, pose0, params, fmt, W, H, K, nframes):
+    """Per-kernel averages over nframes of a fresh run, HIP events on the launch stream."""
+    from azu_amd import native as N
+    from azu_amd.pipeline import FrameLoop
+
+    loop = FrameLoop(params, fmt, W, H, K)
+    loop.reset(pose0)
+    loop.frame(frames_dev[0])
+    N.synchronize()
+    loop.vol.counters(reset=True)
+    N.profile_reset()
+    N.profile_enable(True)
+    for f in range(1, nframes):
+        loop.frame(frames_dev[f])
+    N.synchronize()
+    prof = N.profile_read()
+    N.profile_enable(False)
+    upd, _ = loop.vol.counters(reset=True)
+    loop.close()
+    return prof, upd / max(nframes - 1, 1)
+
+
+def cpu_baseline(frames_mm, pose0, params, K, nframes):
+    import oracle
+
+    v = oracle.Volume(params["resolution"], params["voxel_size"], params["truncation"], params["max_weight"],
+                      params["origin"], params["min_depth"], params["max_depth"], dtype=np.float32)
+    t0 = time.perf_counter()
+    _, ms, _ = v.run_sequence(frames_mm[:nframes], K, pose0)
+    dt = time.perf_counter() - t0
+    v.close()
+    # frame 0 has no ICP; the rate is over frames 1.. (median frame time)
+    med = float(np.median(ms[1:, 4])) if nframes > 1 else float(ms[0, 4])
+    return {"value": 1000.0 / med, "unit": "frames/s", "cores": int(oracle.lib.orc_num_threads()), "kind": "port",
+            "sample": "frames 1..%d of the same 512^3 sequence (%.1f s wall), median frame time %.1f ms "
+                      "(pyramid %.1f, icp %.1f, integrate %.1f, raycast %.1f ms)" % (
+                          nframes - 1, dt, med, np.median(ms[1:, 0]), np.median(ms[1:, 1]),
+                          np.median(ms[1:, 2]), np.median(ms[1:, 3]))}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs the frame loop on a single MI355X "
+              "(the multi-GPU slab decomposition is a design note, not built); use --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    from azu_amd import native as N
+    from azu_amd import scene as sc
+
+    w = sc.WORKLOADS[args.workload]
+    params = sc.volume_params(args.workload)
+    nframes = args.warmup + args.steps
+    t0 = time.perf_counter()
+    frames_mm, poses_true, K = sc.sequence(args.workload, frames=max(nframes, args.profile_frames))
+    log("rendered %d frames in %.1f s" % (len(frames_mm), time.perf_counter() - t0))
+    W, H = w["W"], w["H"]
+    dev = [N.device_array(f) for f in frames_mm]
+    frames_dev = [d.ptr for d in dev]
+    pose0 = poses_true[0]
+    dev_id, dev_name, hbm = N.device_info()
+    log("device", dev_name)
+
+    results = {}
+    forms = {"f16": N.FMT_F16_U16, "f32": N.FMT_F32}
+    for name in (["f16", "f32"] if args.only == "both" else [args.only]):
+        dt, poses = run_form(frames_dev, nframes, pose0, params, forms[name], W, H, K, args.warmup, args.steps)
+        terr = np.abs(poses[:, :3, 3] - poses_true[:nframes, :3, 3]).max(axis=1)
+        results[name] = {"seconds": dt, "fps": args.steps / dt, "ms_per_step": 1000.0 * dt / args.steps,
+                         "max_track_err_m": float(terr.max())}
+        log(name, "%.1f frames/s, %.3f ms/frame, max |t - t_true| %.2f mm" % (
+            results[name]["fps"], results[name]["ms_per_step"], 1000 * terr.max()))
+
+    # Per-kernel timing (fresh run, events on the launch stream) for the roofline.
+    head = "f16" if "f16" in results else "f32"
+    prof, n_upd = profile_form(frames_dev, pose0, params, forms[head], W, H, K,
+                               min(args.profile_frames, len(frames_dev)))
+    nprof = max(min(args.profile_frames, len(frames_dev)) - 1, 1)
+    per_frame = {k: v[1] / nprof for k, v in prof.items()}
+    dom = max(per_frame, key=per_frame.get)
+    launches, total_ms = prof[dom]
+    avg_ms = total_ms / launches
+    s_v = 4 if head == "f16" else 8
+    roof = None
+    if dom.startswith("integrate"):
+        # algorithmic bytes per launch: read + write of every updated voxel, plus the
+        # float depth frame read once (the live depth is float metres on the device)
+        traffic_alg = 2 * s_v * n_upd + 4 * W * H
+        achieved = traffic_alg / (avg_ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                "avg_launch_us": round(avg_ms * 1000, 2), "algorithmic_bytes": int(traffic_alg),
+                "updated_voxels": int(n_upd)}
+    else:
+        roof = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
+                "traffic": None, "kernel": dom, "avg_launch_us": round(avg_ms * 1000, 2)}
+    kern = {k: {"launches": int(v[0]), "avg_us": round(1000 * v[1] / v[0], 2),
+                "us_per_frame": round(1000 * v[1] / nprof, 2)} for k, v in prof.items()}
+    log("kernels:", json.dumps(kern))
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(frames_mm, pose0, params, K, args.cpu_frames)
+        log("cpu baseline", cpu)
+
+    r = results[head]
+    line = {
+        "metric": METRIC,
+        "value": round(r["fps"], 2),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "512^3 TSDF at 2 mm, 640x480 uint16 depth, sphere+plane+10 boxes, orbit+jitter, "
+                        "integrate+raycast+ICP{10,5,4}",
+            "tsdf_storage": "fp16+u16 (4 B/voxel)" if head == "f16" else "fp32+fp32 (8 B/voxel)",
+            "frames": nframes,
+            "device": dev_name,
+        },
+        "fp32_tsdf": ({"value": round(results["f32"]["fps"], 2), "ms_per_step": round(results["f32"]["ms_per_step"], 4),
+                       "max_track_err_m": results["f32"]["max_track_err_m"]} if "f32" in results else None),
+        "fp16_over_fp32": (round(results["f16"]["fps"] / results["f32"]["fps"], 3)
+                           if "f16" in results and "f32" in results else None),
+        "max_track_err_m": r["max_track_err_m"],
+        "roofline": roof,
+        "kernels_us_per_frame": {k: v["us_per_frame"] for k, v in kern.items()},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

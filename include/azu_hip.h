@@ -167,6 +167,9 @@ int azu_tracker_track(azu_tracker* t, const float* d_model_v, const float* d_mod
 /* Same, estimate and reference pose read from device memory when the first kernel runs. */
 int azu_tracker_track_dpose(azu_tracker* t, const float* d_model_v, const float* d_model_n, int MW, int MH,
                             const float* d_pose_estimate, const float* d_ref_pose, azu_stream s);
+/* Sets the tracker's device pose (the one azu_tracker_pose_device points at) without
+ * tracking, e.g. the first frame's pose before any model exists. */
+int azu_tracker_set_pose(azu_tracker* t, const float pose[16], azu_stream s);
 /* Device pointer to the tracked pose (16 floats), valid once the enqueued track ran. */
 int azu_tracker_pose_device(azu_tracker* t, const float** d_pose);
 /* Synchronises the stream and returns the result of the last track. */
