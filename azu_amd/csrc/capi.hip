@@ -42,6 +42,9 @@ using namespace azu;
 
 namespace {
 
+// Integration counters: 256 slot pairs of u64, summed when read (integrate.hip).
+constexpr size_t kCounterBytes = 256 * 2 * sizeof(unsigned long long);
+
 thread_local std::string g_err;
 
 int fail(int code, const std::string& msg) {
@@ -175,7 +178,7 @@ struct azu_volume {
     int mode;
     void* vox = nullptr;
     size_t bytes = 0;
-    unsigned long long* counters = nullptr;  // device [2]: updated, newly observed
+    unsigned long long* counters = nullptr;  // device: 256 slot pairs {updated, newly observed}
     int frames = 0;
     bool exact() const { return mode == AZU_MODE_EXACT || (mode == AZU_MODE_AUTO && format == AZU_FMT_F32); }
 };
@@ -301,7 +304,7 @@ int azu_vol_create(const azu_tsdf_params* p, int format, azu_volume** out) {
         delete v;
         return rc;
     }
-    if (hipMalloc(&v->counters, 2 * sizeof(unsigned long long)) != hipSuccess) {
+    if (hipMalloc(&v->counters, kCounterBytes) != hipSuccess) {
         (void)hipFree(v->vox);
         delete v;
         return fail(AZU_ERR_ALLOC, "counter allocation failed");
@@ -334,7 +337,7 @@ int azu_vol_reset(azu_volume* v, azu_stream s) {
         ProfScope ps("fill_empty", (hipStream_t)s);
         AZU_HIP_TRY(launchFillEmpty(v->format, v->vox, n, (hipStream_t)s), "fill_empty launch");
     }
-    AZU_HIP_TRY(hipMemsetAsync(v->counters, 0, 2 * sizeof(unsigned long long), (hipStream_t)s), "counter reset");
+    AZU_HIP_TRY(hipMemsetAsync(v->counters, 0, kCounterBytes, (hipStream_t)s), "counter reset");
     v->frames = 0;
     return AZU_OK;
 }
@@ -422,12 +425,17 @@ int azu_vol_storage(azu_volume* v, void** d_ptr, size_t* bytes) {
 
 int azu_vol_counters(azu_volume* v, unsigned long long* updated, unsigned long long* newly, int reset) {
     if (!v) return fail(AZU_ERR_ARG, "null volume");
-    unsigned long long h[2] = {0, 0};
+    std::vector<unsigned long long> h(kCounterBytes / sizeof(unsigned long long));
     AZU_HIP_TRY(hipDeviceSynchronize(), "synchronise for counters");
-    AZU_HIP_TRY(hipMemcpy(h, v->counters, sizeof(h), hipMemcpyDeviceToHost), "counter read");
-    if (updated) *updated = h[0];
-    if (newly) *newly = h[1];
-    if (reset) AZU_HIP_TRY(hipMemset(v->counters, 0, sizeof(h)), "counter reset");
+    AZU_HIP_TRY(hipMemcpy(h.data(), v->counters, kCounterBytes, hipMemcpyDeviceToHost), "counter read");
+    unsigned long long u = 0, nw = 0;
+    for (size_t i = 0; i + 1 < h.size(); i += 2) {
+        u += h[i];
+        nw += h[i + 1];
+    }
+    if (updated) *updated = u;
+    if (newly) *newly = nw;
+    if (reset) AZU_HIP_TRY(hipMemset(v->counters, 0, kCounterBytes), "counter reset");
     return AZU_OK;
 }
 

@@ -35,7 +35,7 @@ struct IcpState {
     int converged;
     int counters[5];  // valid_live, valid_model, projected, dist_filtered, angle_filtered
     int stop_level;   // level whose iteration loop has ended, -1 while none
-    int iterations;   // solves run (diagnostic)
+    int iterations;   // accepted steps, one byte per level (level l at bit 8*l)
 };
 
 struct IcpIterArgs {
@@ -222,7 +222,6 @@ __device__ void icpSolve(IcpState* st, const double* sums, int level, int iter) 
         st->stop_level = level;
         return;
     }
-    st->iterations += 1;
     double A[36], b[6];
     int k = 0;
     for (int i = 0; i < 6; ++i)
@@ -314,6 +313,7 @@ __device__ void icpSolve(IcpState* st, const double* sums, int level, int iter) 
         st->pose[12 + i] = nt[i];
     }
     st->final_step = step_norm;
+    st->iterations += 1 << (8 * level);
     st->last_final_step = step_norm;
     st->error = (float)(residual / (double)(inliers > 1 ? inliers : 1));
     st->inliers = inliers;

@@ -74,13 +74,17 @@ __device__ __forceinline__ IntFrame integrateSetup(const VolParams& p, const Pos
     return f;
 }
 
-__device__ __forceinline__ void countWave(unsigned long long* counters, bool upd, bool newly) {
-    if (!counters) return;
-    const unsigned long long bu = __ballot(upd), bn = __ballot(newly);
-    if ((threadIdx.x & 63) == 0 && (bu | bn)) {
-        if (bu) atomicAdd(&counters[0], (unsigned long long)__popcll(bu));
-        if (bn) atomicAdd(&counters[1], (unsigned long long)__popcll(bn));
-    }
+// Updated / newly-observed voxel counts. Each wave keeps its counts in registers and
+// adds them once, at its end, to one of kCounterSlots slot pairs (chosen by block), so
+// no address sees more than a few hundred atomics per frame; readers sum the slots.
+constexpr int kCounterSlots = 256;
+
+__device__ __forceinline__ void flushCounts(unsigned long long* counters, unsigned long long upd,
+                                            unsigned long long newly) {
+    if (!counters || (threadIdx.x & 63) != 0 || (upd | newly) == 0) return;
+    unsigned long long* slot = counters + 2 * (blockIdx.x % kCounterSlots);
+    if (upd) atomicAdd(&slot[0], upd);
+    if (newly) atomicAdd(&slot[1], newly);
 }
 
 constexpr int kChunk = 32;          // x columns per phase
@@ -109,6 +113,7 @@ __global__ __launch_bounds__(256) void integrateExact(typename S::Word* __restri
     float* L = lds[wv];
     const size_t res = (size_t)p.res, plane = res * res;
     const float fW = (float)W, fH = (float)H;
+    unsigned long long n_upd = 0, n_new = 0;  // wave-uniform
 
     for (int xc = f.x0; xc <= f.x1; xc += kChunk) {
         const int n = min(kChunk, f.x1 - xc + 1);
@@ -152,10 +157,12 @@ __global__ __launch_bounds__(256) void integrateExact(typename S::Word* __restri
                 newly = (w_old == 0.0f);
                 *ptr = S::pack((t_old * w_old + tn) / (w_old + 1.0f), smin(w_old + 1.0f, p.max_w));
             }
-            countWave(counters, upd, newly);
+            n_upd += (unsigned long long)__popcll(__ballot(upd));
+            n_new += (unsigned long long)__popcll(__ballot(newly));
         }
         waveSync();
     }
+    flushCounts(counters, n_upd, n_new);
 }
 
 // ---------------------------------------------------------------------------
@@ -197,81 +204,83 @@ template <> struct QuadIO<StoreF32> {
     }
 };
 
+constexpr int kFastZ = 8;  // z slices per block
+
 template <class S>
 __global__ __launch_bounds__(256) void integrateFast(typename S::Word* __restrict__ vox, VolParams p, Intr K, int W,
                                                      int H, const float* __restrict__ depth, float min_d, float max_d,
                                                      PoseArg pa, unsigned long long* __restrict__ counters) {
     const IntFrame f = integrateSetup(p, pa, K, W, H, max_d);
     if (!f.ok) return;
-    const int tx0 = blockIdx.x * 64, ty0 = blockIdx.y * 16, z = blockIdx.z;
-    if (tx0 > f.x1 || tx0 + 63 < f.x0 || ty0 > f.y1 || ty0 + 15 < f.y0 || z < f.z0 || z > f.z1) return;
+    const int tx0 = blockIdx.x * 64, ty0 = blockIdx.y * 16, tz0 = blockIdx.z * kFastZ;
+    if (tx0 > f.x1 || tx0 + 63 < f.x0 || ty0 > f.y1 || ty0 + 15 < f.y0 || tz0 > f.z1 || tz0 + kFastZ - 1 < f.z0)
+        return;
     const int q = threadIdx.x & 15;
     const int y = ty0 + (threadIdx.x >> 4);
     const int xq = tx0 + q * 4;
     const bool row_ok = y >= f.y0 && y <= f.y1 && y < p.res;
     const float vs = p.vs;
-    // Camera coordinate of voxel x: c(x) = base + x * ax, with the projection numerators
-    // u_num = fx c.x + (cx + 1/2) c.z and v_num = fy c.y + (cy + 1/2) c.z affine in x too.
-    const F3 pw{p.origin[0], p.origin[1] + (float)y * vs, p.origin[2] + (float)z * vs};
-    const F3 base = f3add(mulv(f.R_wc, pw), f.t_wc);
-    const F3 ax = f.step;
     const float cu = K.cx + 0.5f, cv = K.cy + 0.5f;
-    const float bu = fmaf(K.fx, base.x, cu * base.z), bv = fmaf(K.fy, base.y, cv * base.z);
+    const F3 ax = f.step;
+    // Camera coordinate of voxel (x, y, z): c = base(y, z) + x * ax, and the projection
+    // numerators u_num = fx c.x + (cx + 1/2) c.z, v_num = fy c.y + (cy + 1/2) c.z are
+    // affine in x as well: u = floor(u_num / c.z).
     const float su = fmaf(K.fx, ax.x, cu * ax.z), sv = fmaf(K.fy, ax.y, cv * ax.z);
+    const F3 az{f.R_wc.m[0][2] * vs, f.R_wc.m[1][2] * vs, f.R_wc.m[2][2] * vs};
+    const F3 pw{p.origin[0], p.origin[1] + (float)y * vs, p.origin[2] + (float)tz0 * vs};
+    F3 base = f3add(mulv(f.R_wc, pw), f.t_wc);
     const float inv_trunc = 1.0f / p.trunc, neg_trunc = -p.trunc;
-    float tn[4];
-    bool any = false;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        tn[i] = __builtin_nanf("");
-        const int x = xq + i;
-        if (!row_ok || x < f.x0 || x > f.x1) continue;
-        const float fx = (float)x;
-        const float cz = fmaf(fx, ax.z, base.z);
-        if (!(cz > 0.0f)) continue;
-        const float rz = __builtin_amdgcn_rcpf(cz);
-        const float uf = floorf(fmaf(fx, su, bu) * rz);
-        const float vf = floorf(fmaf(fx, sv, bv) * rz);
-        if (!(uf >= 0.0f && uf < (float)W && vf >= 0.0f && vf < (float)H)) continue;
-        const float d = depth[(int)vf * W + (int)uf];
-        if (!(d >= min_d && d <= max_d)) continue;
-        const float sdf = d - cz;
-        if (sdf < neg_trunc) continue;
-        tn[i] = fminf(1.0f, sdf * inv_trunc);
-        any = true;
-    }
-    bool upd_any = false;
+    const int zend = min(tz0 + kFastZ - 1, f.z1);
     int nupd = 0, nnew = 0;
-    if (any) {
-        typename S::Word* ptr = vox + ((size_t)z * p.res + (size_t)y) * (size_t)p.res + (size_t)xq;
-        typename QuadIO<S>::Q qv = QuadIO<S>::load(ptr);
+    for (int z = tz0; z <= zend; ++z, base = f3add(base, az)) {
+        if (z < f.z0) continue;
+        const float bu = fmaf(K.fx, base.x, cu * base.z), bv = fmaf(K.fy, base.y, cv * base.z);
+        float tn[4];
+        bool any = false;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            if (__builtin_isnan(tn[i])) continue;
-            const typename S::Word wd = QuadIO<S>::get(qv, i);
-            const float w_old = S::weight(wd);
-            const float t_old = S::tsdf(wd);
-            const float t_new = fmaf(t_old, w_old, tn[i]) * __builtin_amdgcn_rcpf(w_old + 1.0f);
-            QuadIO<S>::set(qv, i, S::pack(t_new, fminf(w_old + 1.0f, p.max_w)));
-            ++nupd;
-            nnew += (w_old == 0.0f);
+            tn[i] = __builtin_nanf("");
+            const int x = xq + i;
+            if (!row_ok || x < f.x0 || x > f.x1) continue;
+            const float fx = (float)x;
+            const float cz = fmaf(fx, ax.z, base.z);
+            if (!(cz > 0.0f)) continue;
+            const float rz = __builtin_amdgcn_rcpf(cz);
+            const float uf = floorf(fmaf(fx, su, bu) * rz);
+            const float vf = floorf(fmaf(fx, sv, bv) * rz);
+            if (!(uf >= 0.0f && uf < (float)W && vf >= 0.0f && vf < (float)H)) continue;
+            const float d = depth[(int)vf * W + (int)uf];
+            if (!(d >= min_d && d <= max_d)) continue;
+            const float sdf = d - cz;
+            if (sdf < neg_trunc) continue;
+            tn[i] = fminf(1.0f, sdf * inv_trunc);
+            any = true;
         }
-        QuadIO<S>::store(ptr, qv);
-        upd_any = true;
+        if (any) {
+            typename S::Word* ptr = vox + ((size_t)z * p.res + (size_t)y) * (size_t)p.res + (size_t)xq;
+            typename QuadIO<S>::Q qv = QuadIO<S>::load(ptr);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                if (__builtin_isnan(tn[i])) continue;
+                const typename S::Word wd = QuadIO<S>::get(qv, i);
+                const float w_old = S::weight(wd);
+                const float t_old = S::tsdf(wd);
+                const float t_new = fmaf(t_old, w_old, tn[i]) * __builtin_amdgcn_rcpf(w_old + 1.0f);
+                QuadIO<S>::set(qv, i, S::pack(t_new, fminf(w_old + 1.0f, p.max_w)));
+                ++nupd;
+                nnew += (w_old == 0.0f);
+            }
+            QuadIO<S>::store(ptr, qv);
+        }
     }
     if (counters) {
-        // Per-wave sums of the per-thread counts.
-        int a = nupd, b = nnew;
+        unsigned long long a = (unsigned long long)nupd, b = (unsigned long long)nnew;
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) {
             a += __shfl_xor(a, o);
             b += __shfl_xor(b, o);
         }
-        const bool anyw = __ballot(upd_any) != 0;
-        if ((threadIdx.x & 63) == 0 && anyw) {
-            atomicAdd(&counters[0], (unsigned long long)a);
-            if (b) atomicAdd(&counters[1], (unsigned long long)b);
-        }
+        flushCounts(counters, a, b);
     }
 }
 
@@ -301,7 +310,7 @@ hipError_t launchIntegrate(int format, bool exact, void* vox, const VolParams& p
             hipLaunchKernelGGL(integrateExact<StoreF16>, dim3(blocks), dim3(256), 0, st, (uint32_t*)vox, p, K, W,
                                H, depth, min_d, max_d, pa, counters);
     } else {
-        const dim3 grid((p.res + 63) / 64, (p.res + 15) / 16, p.res);
+        const dim3 grid((p.res + 63) / 64, (p.res + 15) / 16, (p.res + kFastZ - 1) / kFastZ);
         if (format == kFmtF32)
             hipLaunchKernelGGL(integrateFast<StoreF32>, grid, dim3(256), 0, st, (float2*)vox, p, K, W, H, depth,
                                min_d, max_d, pa, counters);

@@ -83,7 +83,7 @@ typedef struct azu_icp_result {
     int projected_points;
     int dist_filtered;
     int angle_filtered;
-    int iterations; /* solves run over all levels (diagnostic) */
+    int iterations; /* accepted steps, one byte per level (level l at bit 8*l) */
 } azu_icp_result;
 
 typedef struct azu_volume azu_volume;

@@ -50,6 +50,7 @@ lib.orc_floor_to_int.restype = I
 lib.orc_floor_to_int.argtypes = [F, P]
 lib.orc_scale_intrinsics.argtypes = [P, I, I, I, I, P]
 lib.orc_num_threads.restype = I
+lib.orc_set_icp_chunks.argtypes = [I]
 lib.orc_mc_extract.restype = LL
 lib.orc_mc_extract.argtypes = [P, P, I, F, P, LL, P, P, P, P, P, P]
 
@@ -202,13 +203,13 @@ def icp_track(levels_v, levels_n, mv, mn, K, est, ref, iters=(10, 5, 4), dist=0.
     it = np.asarray(iters, np.int32)
     pose = np.empty(16, dtype)
     sc = np.empty(2, dtype)
-    ints = np.empty(8, np.int32)
+    ints = np.empty(9, np.int32)
     info = np.empty(36, dtype)
     getattr(lib, "orc_icp_track_" + sfx)(_d(lv[0]), _d(ln[0]), _d(lv[1]), _d(ln[1]), _d(lv[2]), _d(ln[2]), W, H,
                                           _d(mv), _d(mn), MW, MH, _d(Kc), _d(cm(est)), _d(cm(ref)), _d(it), dist,
                                           angle, _d(pose), _d(sc), _d(ints), _d(info))
     keys = ["inliers", "converged", "tracking_ok", "valid_live_points", "valid_model_points", "projected_points",
-            "dist_filtered", "angle_filtered"]
+            "dist_filtered", "angle_filtered", "steps"]
     r = dict(zip(keys, ints.tolist()))
     r.update(pose=rm(pose), error=float(sc[0]), final_step=float(sc[1]), information=info.reshape(6, 6))
     return r

@@ -527,6 +527,7 @@ template <class T> struct IcpResult {
     T final_step;
     T information[36];
     int valid_live, valid_model, projected, dist_filtered, angle_filtered;
+    int steps;  // accepted steps, packed one byte per level (level l at bit 8*l)
 };
 
 // One live pyramid level: W x H vertices / normals in camera coordinates.
@@ -538,11 +539,15 @@ template <class T> struct Level {
 
 constexpr float kHuberK = 0.02f;
 
-// Fixed virtual-thread partition of rows [1, H-1) into kIcpChunks contiguous
+// Fixed virtual-thread partition of rows [1, H-1) into icpChunks() contiguous
 // blocks, folded in block order (the reference's schedule(static) + ordered fold,
 // ICPTracker.cpp:228, 334-351, with the thread count pinned so the sum order does
-// not depend on the host).
-constexpr int kIcpChunks = 8;
+// not depend on the host). Default 8; settable to model other reference thread counts.
+inline int& icpChunks() {
+    static int n = 8;
+    return n;
+}
+constexpr int kIcpMaxChunks = 64;
 
 struct IcpSystemCounters {
     int valid_live, valid_model, projected, dist_filtered, angle_filtered, inliers;
@@ -568,7 +573,8 @@ bool buildLinearSystem(const Level<T>& live, const V3<T>* mv, const V3<T>* mn, i
         T A[21], b[6], res;
         int count, vl, vm, pr, df, af;
     };
-    Acc acc[kIcpChunks];
+    const int nch = std::max(1, std::min(icpChunks(), kIcpMaxChunks));
+    Acc acc[kIcpMaxChunks];
     for (auto& a : acc) {
         for (T& x : a.A) x = 0;
         for (T& x : a.b) x = 0;
@@ -577,8 +583,8 @@ bool buildLinearSystem(const Level<T>& live, const V3<T>* mv, const V3<T>* mn, i
     }
     const int rows = std::max(0, H - 2);
 #pragma omp parallel for schedule(static, 1)
-    for (int ch = 0; ch < kIcpChunks; ++ch) {
-        const int base = rows / kIcpChunks, extra = rows % kIcpChunks;
+    for (int ch = 0; ch < nch; ++ch) {
+        const int base = rows / nch, extra = rows % nch;
         const int r0 = 1 + ch * base + std::min(ch, extra);
         const int r1 = r0 + base + (ch < extra ? 1 : 0);
         Acc& a = acc[ch];
@@ -638,7 +644,8 @@ bool buildLinearSystem(const Level<T>& live, const V3<T>* mv, const V3<T>* mn, i
     for (int i = 0; i < 6; ++i) b[i] = 0;
     residual = 0;
     cnt = IcpSystemCounters{0, 0, 0, 0, 0, 0};
-    for (const Acc& a : acc) {
+    for (int ch = 0; ch < nch; ++ch) {
+        const Acc& a = acc[ch];
         int k = 0;
         for (int i = 0; i < 6; ++i) {
             for (int j = i; j < 6; ++j) {
@@ -894,6 +901,7 @@ IcpResult<T> trackLevel(const Level<T>& live, const V3<T>* mv, const V3<T>* mn, 
             for (int j = 0; j < 3; ++j) np.R.m[i][j] = static_cast<T>(Rp[i * 3 + j]);
         pose = np;
         r.final_step = step_norm;
+        r.steps += 1 << (8 * level);
         r.error = residual / static_cast<T>(std::max(c.inliers, 1));
         r.inliers = c.inliers;
         if (step_norm < static_cast<T>(5e-5f)) {
@@ -919,7 +927,9 @@ IcpResult<T> track(const Level<T> levels[3], const V3<T>* mv, const V3<T>* mn, i
         IcpResult<T> lr = trackLevel(levels[level], mv, mn, MW, MH, K, cur, ref, prm, level, prm.iters[level]);
         if (std::isfinite(lr.final_step)) last_final = lr.final_step;
         lr.final_step = last_final;
+        const int steps_before = result.steps;
         result = lr;
+        result.steps += steps_before;
     }
     bool fin = true;
     for (T v : result.pose) fin = fin && std::isfinite(v);

@@ -165,7 +165,7 @@ template <class T> struct Pyr {
         res_scalars[1] = r.final_step;                                                                       \
         res_ints[0] = r.inliers; res_ints[1] = r.converged; res_ints[2] = r.tracking_ok;                     \
         res_ints[3] = r.valid_live; res_ints[4] = r.valid_model; res_ints[5] = r.projected;                  \
-        res_ints[6] = r.dist_filtered; res_ints[7] = r.angle_filtered;                                       \
+        res_ints[6] = r.dist_filtered; res_ints[7] = r.angle_filtered; res_ints[8] = r.steps;                \
         for (int i = 0; i < 36; ++i) res_info[i] = r.information[i];                                         \
     }                                                                                                        \
     /* The benchmark frame loop over a uint16 mm sequence (the pipeline order): frame 0 integrates at the    \
@@ -246,6 +246,7 @@ ORC_API void orc_scale_intrinsics(const float* K, int rw, int rh, int w, int h, 
     const Intr k = scaleIntrinsics(mkK(K), rw, rh, w, h);
     out[0] = k.fx; out[1] = k.fy; out[2] = k.cx; out[3] = k.cy;
 }
+ORC_API void orc_set_icp_chunks(int n) { icpChunks() = n; }
 ORC_API int orc_num_threads() {
 #ifdef _OPENMP
     return omp_get_max_threads();

@@ -44,3 +44,19 @@ def pose_diff(Ta, Tb):
     s = 0.5 * np.array([dR[2, 1] - dR[1, 2], dR[0, 2] - dR[2, 0], dR[1, 0] - dR[0, 1]])
     ang = float(np.arctan2(np.linalg.norm(s), 0.5 * (np.trace(dR) - 1.0)))
     return dt, ang
+
+
+def steps_per_level(packed):
+    return tuple((int(packed) >> (8 * l)) & 255 for l in range(3))
+
+
+def icp_pose_parity(gpu_pose, gpu_steps, ref_pose, ref_steps):
+    """The ICP parity rule. When both runs take the same accepted steps per level the
+    poses agree within 1e-5 m / 1e-5 rad. The stop test |x| < 5e-5 is discrete, and the
+    reference's own result moves by one step when its input moves by one float ulp, so
+    when the step counts differ the poses must agree within one step: 1e-4 m / 1e-4 rad.
+    Returns (ok, dt, ang, same_steps)."""
+    dt, ang = pose_diff(gpu_pose, ref_pose)
+    same = steps_per_level(gpu_steps) == steps_per_level(ref_steps)
+    tol = 1e-5 if same else 1e-4
+    return (dt < tol and ang < tol), dt, ang, same
