@@ -34,6 +34,7 @@ hipError_t launchIcpIter(const float* lv, const float* ln, int W, int H, const f
                          double* partials, unsigned* ticket, hipStream_t s);
 void icpStateRead(const void* hostCopy, float* pose, float* information, float* error, float* final_step,
                   int* inliers, int* converged, int* counters, int* iterations);
+hipError_t launchDownsample(int W, int H, float* const d[3], float* const v[3], float* const n[3], hipStream_t st);
 hipError_t launchVolConvert(int format, bool to_float, void* vox, float* tsdf, float* weight, size_t n,
                             hipStream_t st);
 }  // namespace azu
@@ -574,6 +575,24 @@ static int prepareImpl(azu_tracker* t, const void* src, bool mm, azu_stream s) {
 }
 int azu_tracker_prepare_mm(azu_tracker* t, const uint16_t* d, azu_stream s) { return prepareImpl(t, d, true, s); }
 int azu_tracker_prepare_m(azu_tracker* t, const float* d, azu_stream s) { return prepareImpl(t, d, false, s); }
+
+int azu_tracker_load_level(azu_tracker* t, int level, const float* d_depth, const float* d_vertices,
+                           const float* d_normals, azu_stream s) {
+    if (!t || level < 0 || level > 2 || !d_vertices || !d_normals) return fail(AZU_ERR_ARG, "bad tracker, level or buffer");
+    const size_t np = (size_t)t->lw[level] * t->lh[level];
+    const hipStream_t st = (hipStream_t)s;
+    if (d_depth) AZU_HIP_TRY(hipMemcpyAsync(t->d[level], d_depth, np * 4, hipMemcpyDeviceToDevice, st), "level copy");
+    AZU_HIP_TRY(hipMemcpyAsync(t->v[level], d_vertices, np * 12, hipMemcpyDeviceToDevice, st), "level copy");
+    AZU_HIP_TRY(hipMemcpyAsync(t->n[level], d_normals, np * 12, hipMemcpyDeviceToDevice, st), "level copy");
+    return AZU_OK;
+}
+
+int azu_tracker_downsample(azu_tracker* t, azu_stream s) {
+    if (!t) return fail(AZU_ERR_ARG, "null tracker");
+    ProfScope ps("live_pyramid", (hipStream_t)s);
+    AZU_HIP_TRY(launchDownsample(t->W, t->H, t->d, t->v, t->n, (hipStream_t)s), "downsample launch");
+    return AZU_OK;
+}
 
 int azu_tracker_level(azu_tracker* t, int level, const float** d, const float** v, const float** n, int* W, int* H) {
     if (!t || level < 0 || level > 2) return fail(AZU_ERR_ARG, "bad tracker or level");

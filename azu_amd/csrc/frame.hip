@@ -177,3 +177,15 @@ hipError_t launchLive(bool mm, const void* src, int W, int H, Intr K, float min_
 }
 
 }  // namespace azu
+
+namespace azu {
+
+hipError_t launchDownsample(int W, int H, float* const d[3], float* const v[3], float* const n[3], hipStream_t st) {
+    const int W1 = W / 2, H1 = H / 2;
+    const dim3 g1(((W1 + 1) / 2 + 15) / 16, ((H1 + 1) / 2 + 15) / 16);
+    hipLaunchKernelGGL(livePyramid, g1, dim3(256), 0, st, W, H, d[0], v[0], n[0], d[1], v[1], n[1], d[2], v[2],
+                       n[2]);
+    return hipGetLastError();
+}
+
+}  // namespace azu

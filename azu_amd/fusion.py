@@ -194,6 +194,16 @@ class ICPTracker:
     def prepare_m_device(self, d_m, stream=None):
         check(lib.azu_tracker_prepare_m(self.h, d_m, stream))
 
+    def load_level(self, l, vertices, normals, depth=None):
+        """Load one live pyramid level from host arrays (the reference's FramePyramid)."""
+        bufs = [N.device_array(N.f32(vertices)), N.device_array(N.f32(normals))]
+        dd = N.device_array(N.f32(depth)) if depth is not None else None
+        check(lib.azu_tracker_load_level(self.h, l, dd.ptr if dd else None, bufs[0].ptr, bufs[1].ptr, None))
+        N.synchronize()
+
+    def downsample(self):
+        check(lib.azu_tracker_downsample(self.h, None))
+
     def level(self, l):
         d = ctypes.c_void_p()
         v = ctypes.c_void_p()

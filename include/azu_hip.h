@@ -156,6 +156,13 @@ int azu_tracker_set_params(azu_tracker* t, const azu_icp_params* p);
  * uint16 millimetres (x 0.001 m) or float metres, band-gated to [min_depth, max_depth]. */
 int azu_tracker_prepare_mm(azu_tracker* t, const uint16_t* d_depth_mm, azu_stream s);
 int azu_tracker_prepare_m(azu_tracker* t, const float* d_depth_m, azu_stream s);
+/* ICPTracker::track takes a host FramePyramid (ICPTracker.h:91-94); these load one
+ * level of it from device buffers instead (copies; d_depth may be NULL for levels the
+ * ICP reads only as vertices / normals), and rebuild levels 1 and 2 from level 0 with
+ * the buildFramePyramid downsample (FrameData.cpp:262-345). */
+int azu_tracker_load_level(azu_tracker* t, int level, const float* d_depth, const float* d_vertices,
+                           const float* d_normals, azu_stream s);
+int azu_tracker_downsample(azu_tracker* t, azu_stream s);
 /* Device buffers of pyramid level 0..2 (depth metres, packed vertices / normals). */
 int azu_tracker_level(azu_tracker* t, int level, const float** d_depth, const float** d_vertices,
                       const float** d_normals, int* W, int* H);

@@ -735,9 +735,11 @@ template <class T> inline bool ldltSolve(const T Ain[36], const T bin[6], T x[6]
         }
         const T d = a[k * 6 + k];
         if (d == T(0) || !std::isfinite(d)) return false;
+        T col[6];
+        for (int i = k + 1; i < 6; ++i) col[i] = a[i * 6 + k];
         for (int i = k + 1; i < 6; ++i) {
-            const T l = a[i * 6 + k] / d;
-            for (int j = k + 1; j <= i; ++j) a[i * 6 + j] -= l * a[j * 6 + k];
+            const T l = col[i] / d;
+            for (int j = k + 1; j <= i; ++j) a[i * 6 + j] -= l * col[j];
             a[i * 6 + k] = l;
         }
         for (int i = k + 1; i < 6; ++i)

@@ -247,6 +247,12 @@ ORC_API void orc_scale_intrinsics(const float* K, int rw, int rh, int w, int h, 
     out[0] = k.fx; out[1] = k.fy; out[2] = k.cx; out[3] = k.cy;
 }
 ORC_API void orc_set_icp_chunks(int n) { icpChunks() = n; }
+// One downsample step (FrameData.cpp:262-334) on explicit level arrays.
+ORC_API void orc_downsample_f32(const float* d, const float* v, const float* n, int W, int H, float* d1, float* v1,
+                                float* n1) {
+    downsample(d, reinterpret_cast<const V3<float>*>(v), reinterpret_cast<const V3<float>*>(n), W, H, d1,
+               reinterpret_cast<V3<float>*>(v1), reinterpret_cast<V3<float>*>(n1));
+}
 ORC_API int orc_num_threads() {
 #ifdef _OPENMP
     return omp_get_max_threads();
