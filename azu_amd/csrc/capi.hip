@@ -29,6 +29,13 @@ size_t icpStateBytes();
 int icpBlocksFor(int W, int H);
 size_t icpPartialsBytes(int maxBlocks);
 hipError_t launchIcpInit(void* st, const PoseArg& est, const PoseArg& ref, hipStream_t s);
+hipError_t launchPredict(const float* cur, float* last, float* pred, hipStream_t s);
+hipError_t launchIcpClearTicks(void* st, hipStream_t s);
+int depthLossBlocks(int W, int H);
+hipError_t launchDepthLoss(int format, const void* vox, const VolParams& p, const PoseArg& pa, Intr K, int W, int H,
+                           const float* d_obs, float step, int backward, float* grad_tsdf, float* depth_out,
+                           double* partials, double* out8, hipStream_t st);
+void icpTicksRead(const void* hostCopy, unsigned long long* ticks);
 hipError_t launchIcpIter(const float* lv, const float* ln, int W, int H, const float* mv, const float* mn, int MW,
                          int MH, Intr Km, float angle_cos, float dist_sq, int level, int iter, void* st,
                          double* partials, unsigned* ticket, hipStream_t s);
@@ -180,6 +187,9 @@ struct azu_volume {
     void* vox = nullptr;
     size_t bytes = 0;
     unsigned long long* counters = nullptr;  // device: 256 slot pairs {updated, newly observed}
+    double* adj_partials = nullptr;          // depth-loss block partials
+    size_t adj_bytes = 0;
+    double* adj_out = nullptr;               // depth-loss result [8]
     int frames = 0;
     bool exact() const { return mode == AZU_MODE_EXACT || (mode == AZU_MODE_AUTO && format == AZU_FMT_F32); }
 };
@@ -196,6 +206,7 @@ struct azu_tracker {
     double* partials = nullptr;
     unsigned* ticket = nullptr;
     void* host_state = nullptr;  // pinned
+    float* motion = nullptr;     // device: previous pose [16], predicted pose [16]
     bool tracked = false;
 };
 
@@ -327,6 +338,8 @@ int azu_vol_destroy(azu_volume* v) {
     if (!v) return AZU_OK;
     if (v->vox) (void)hipFree(v->vox);
     if (v->counters) (void)hipFree(v->counters);
+    if (v->adj_partials) (void)hipFree(v->adj_partials);
+    if (v->adj_out) (void)hipFree(v->adj_out);
     delete v;
     return AZU_OK;
 }
@@ -503,6 +516,36 @@ int azu_raycast_dpose(azu_volume* v, const float* d_pose, const float K[4], int 
     return raycastImpl(v, poseByPtr(d_pose), K, W, H, d_vertices, d_normals, d_t_hit, s);
 }
 
+int azu_raycast_depth_loss(azu_volume* v, const float pose[16], const float K[4], int W, int H,
+                           const float* d_depth_obs, float step_voxels, int backward, float* d_grad_tsdf,
+                           float* d_depth_out, double out[8], azu_stream s) {
+    if (!v || !pose || !out) return fail(AZU_ERR_ARG, "null volume, pose or output");
+    if (W <= 0 || H <= 0) return fail(AZU_ERR_ARG, "image size must be positive");
+    if (!finiteK(K)) return fail(AZU_ERR_ARG, "intrinsics must be finite with non-zero focal lengths");
+    if (!(step_voxels >= 0.05f && step_voxels <= 8.0f)) return fail(AZU_ERR_ARG, "step_voxels must be in [0.05, 8]");
+    const hipStream_t st = (hipStream_t)s;
+    const size_t need = sizeof(double) * 8 * (size_t)depthLossBlocks(W, H);
+    if (v->adj_bytes < need) {
+        if (v->adj_partials) (void)hipFree(v->adj_partials);
+        v->adj_partials = nullptr;
+        v->adj_bytes = 0;
+        if (hipMalloc((void**)&v->adj_partials, need) != hipSuccess) return fail(AZU_ERR_ALLOC, "adjoint scratch");
+        v->adj_bytes = need;
+    }
+    if (!v->adj_out && hipMalloc((void**)&v->adj_out, 8 * sizeof(double)) != hipSuccess)
+        return fail(AZU_ERR_ALLOC, "adjoint output");
+    {
+        ProfScope ps(backward ? "depth_loss_bwd" : "depth_loss_fwd", st);
+        AZU_HIP_TRY(launchDepthLoss(v->format, v->vox, toVol(v->params), poseByValue(pose),
+                                    Intr{K[0], K[1], K[2], K[3]}, W, H, d_depth_obs, step_voxels * v->params.voxel_size,
+                                    backward, d_grad_tsdf, d_depth_out, v->adj_partials, v->adj_out, st),
+                    "depth loss launch");
+    }
+    AZU_HIP_TRY(hipMemcpyAsync(out, v->adj_out, 8 * sizeof(double), hipMemcpyDeviceToHost, st), "loss read");
+    AZU_HIP_TRY(hipStreamSynchronize(st), "loss sync");
+    return AZU_OK;
+}
+
 // ---- tracker ---------------------------------------------------------------------
 int azu_tracker_destroy(azu_tracker* t) {
     if (!t) return AZU_OK;
@@ -515,6 +558,7 @@ int azu_tracker_destroy(azu_tracker* t) {
     if (t->partials) (void)hipFree(t->partials);
     if (t->ticket) (void)hipFree(t->ticket);
     if (t->host_state) (void)hipHostFree(t->host_state);
+    if (t->motion) (void)hipFree(t->motion);
     delete t;
     return AZU_OK;
 }
@@ -546,6 +590,7 @@ int azu_tracker_create(int W, int H, const float K[4], const azu_icp_params* p, 
     ok = ok && hipMalloc((void**)&t->partials, icpPartialsBytes(1024)) == hipSuccess;
     ok = ok && hipMalloc((void**)&t->ticket, sizeof(unsigned)) == hipSuccess;
     ok = ok && hipHostMalloc(&t->host_state, icpStateBytes(), hipHostMallocDefault) == hipSuccess;
+    ok = ok && hipMalloc((void**)&t->motion, 32 * sizeof(float)) == hipSuccess;
     if (!ok) {
         azu_tracker_destroy(t);
         return fail(AZU_ERR_ALLOC, "tracker allocation failed");
@@ -646,6 +691,17 @@ int azu_tracker_set_pose(azu_tracker* t, const float pose[16], azu_stream s) {
     if (!t || !pose) return fail(AZU_ERR_ARG, "null argument");
     ProfScope ps("icp_init", (hipStream_t)s);
     AZU_HIP_TRY(launchIcpInit(t->state, poseByValue(pose), poseByValue(pose), (hipStream_t)s), "pose set launch");
+    // the motion model's previous pose starts equal to the current one (zero motion)
+    AZU_HIP_TRY(hipMemcpyAsync(t->motion, t->state, 64, hipMemcpyDeviceToDevice, (hipStream_t)s), "pose set copy");
+    return AZU_OK;
+}
+
+int azu_tracker_predict(azu_tracker* t, const float** d_predicted, azu_stream s) {
+    if (!t) return fail(AZU_ERR_ARG, "null tracker");
+    ProfScope ps("predict_pose", (hipStream_t)s);
+    AZU_HIP_TRY(launchPredict(static_cast<const float*>(t->state), t->motion, t->motion + 16, (hipStream_t)s),
+                "predict launch");
+    if (d_predicted) *d_predicted = t->motion + 16;
     return AZU_OK;
 }
 
@@ -673,6 +729,22 @@ int azu_tracker_result(azu_tracker* t, azu_icp_result* out, azu_stream s) {
     for (float x : out->pose) fin = fin && std::isfinite(x);
     // ICPTracker::track's success rule (ICPTracker.cpp:62-64)
     out->tracking_ok = (fin && out->inliers > 100 && (out->converged || out->final_step <= 1e-3f)) ? 1 : 0;
+    return AZU_OK;
+}
+
+int azu_tracker_phase_times(azu_tracker* t, double out[3], int reset) {
+    if (!t || !out) return fail(AZU_ERR_ARG, "null argument");
+    AZU_HIP_TRY(hipDeviceSynchronize(), "synchronise for phase times");
+    AZU_HIP_TRY(hipMemcpy(t->host_state, t->state, icpStateBytes(), hipMemcpyDeviceToHost), "state read");
+    unsigned long long ticks[3];
+    icpTicksRead(t->host_state, ticks);
+    out[0] = ticks[0] * 1e-5;  // s_memrealtime runs at 100 MHz
+    out[1] = ticks[1] * 1e-5;
+    out[2] = (double)ticks[2];
+    if (reset) {
+        AZU_HIP_TRY(launchIcpClearTicks(t->state, nullptr), "tick clear launch");
+        AZU_HIP_TRY(hipDeviceSynchronize(), "tick clear");
+    }
     return AZU_OK;
 }
 

@@ -3,16 +3,17 @@
 // constants and predicates of include/tracking/ICPShared.h:34-110).
 //
 // One kernel launch per ICP iteration, no host round trip:
-//   1. every thread associates its live pixels with the model image, builds
-//      J = [R^T n_m ; v x R^T n_m], the Huber weight and loss, and accumulates the
-//      21 upper-triangle A terms, 6 b terms and the residual in FP32 with Kahan
-//      (TwoSum) compensation, plus 6 integer counters;
-//   2. the block folds its lanes with wave shuffles and LDS, still as compensated
-//      FP32 pairs, and writes one FP64 partial per slot;
+//   1. every thread walks its live pixels (grid-stride), associates each with the model
+//      image, builds J = [R^T n_m ; v x R^T n_m], the Huber weight and loss, and
+//      accumulates the 21 upper-triangle A terms, 6 b terms and the residual in FP32
+//      with Kahan (TwoSum) compensation, plus 6 integer counters;
+//   2. the block folds its lanes with DPP wave reductions and LDS, still as
+//      compensated FP32 pairs, and writes one FP64 partial per slot;
 //   3. the last block to finish (atomic ticket, agent-scope fences) sums the partials
-//      in FP64 in block order (deterministic), and one lane runs the solve: adaptive
-//      damping from the extreme eigenvalues, FP64 Cholesky of A + lambda I, step caps,
-//      Rodrigues update, pose * delta, re-projection onto SO(3), convergence test.
+//      in FP64 in a fixed order (deterministic), issuing all of its loads at once, and
+//      one lane runs the solve: adaptive damping from the extreme eigenvalues, FP64
+//      Cholesky of A + lambda I, step caps, Rodrigues update, pose * delta,
+//      re-projection onto SO(3), convergence test.
 // The tracker state (pose, per-level result, stop flag) lives in device memory, so
 // all 19 iterations of a frame are enqueued back to back; once a level has stopped,
 // its remaining launches return at once.
@@ -23,6 +24,8 @@ namespace azu {
 constexpr int kIcpSlotsF = 28;  // 21 A + 6 b + residual
 constexpr int kIcpSlotsI = 6;   // inliers, valid_live, valid_model, projected, dist_f, angle_f
 constexpr int kIcpSlots = kIcpSlotsF + kIcpSlotsI;
+constexpr int kIcpMaxBlocks = 256;
+constexpr int kSumGroups = 7;  // last block: 7 threads per slot (7 * 34 = 238 <= 256)
 
 struct IcpState {
     float pose[16];
@@ -36,6 +39,9 @@ struct IcpState {
     int counters[5];  // valid_live, valid_model, projected, dist_filtered, angle_filtered
     int stop_level;   // level whose iteration loop has ended, -1 while none
     int iterations;   // accepted steps, one byte per level (level l at bit 8*l)
+    // Phase timing of the last block, in 100 MHz clock ticks summed over the launches
+    // that reached it: cross-block sum, solve; and the number of such launches.
+    unsigned long long ticks_sum, ticks_solve, ticks_n;
 };
 
 struct IcpIterArgs {
@@ -51,12 +57,51 @@ struct IcpIterArgs {
     int level, iter;
 };
 
+// Compensated (TwoSum) accumulation of the pair (b_hi, b_lo) into (hi, lo).
 __device__ __forceinline__ void twoSum(float& hi, float& lo, float b_hi, float b_lo) {
     const float s = hi + b_hi;
     const float bp = s - hi;
     const float e = (hi - (s - bp)) + (b_hi - bp);
     hi = s;
     lo = lo + b_lo + e;
+}
+
+// DPP lane move: lanes whose source is out of range or in a masked row read 0, which
+// is the identity of both sums below.
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ float dppF(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, ROW_MASK,
+                                                                 0xf, true));
+}
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ int dppI(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, ROW_MASK, 0xf, true);
+}
+
+// Wave64 reduction to lane 63 with the gfx9 DPP sequence: quad_perm [1,0,3,2],
+// quad_perm [2,3,0,1], row_shr:4, row_shr:8, row_bcast:15 (rows 1, 3), row_bcast:31
+// (rows 2, 3).
+template <int CTRL, int RM>
+__device__ __forceinline__ void dppStepPair(float& hi, float& lo) {
+    const float ph = dppF<CTRL, RM>(hi), pl = dppF<CTRL, RM>(lo);
+    twoSum(hi, lo, ph, pl);
+}
+__device__ __forceinline__ void waveSumPair(float& hi, float& lo) {
+    dppStepPair<0xb1, 0xf>(hi, lo);
+    dppStepPair<0x4e, 0xf>(hi, lo);
+    dppStepPair<0x114, 0xf>(hi, lo);
+    dppStepPair<0x118, 0xf>(hi, lo);
+    dppStepPair<0x142, 0xa>(hi, lo);
+    dppStepPair<0x143, 0xc>(hi, lo);
+}
+__device__ __forceinline__ int waveSumInt(int v) {
+    v += dppI<0xb1, 0xf>(v);
+    v += dppI<0x4e, 0xf>(v);
+    v += dppI<0x114, 0xf>(v);
+    v += dppI<0x118, 0xf>(v);
+    v += dppI<0x142, 0xa>(v);
+    v += dppI<0x143, 0xc>(v);
+    return v;
 }
 
 // ---------------------------------------------------------------------------
@@ -68,11 +113,12 @@ __device__ bool cholesky6(const double* A, double* L) {
         for (int k = 0; k < j; ++k) s -= L[j * 6 + k] * L[j * 6 + k];
         if (!(s > 0.0)) return false;
         const double d = sqrt(s);
+        const double inv = 1.0 / d;
         L[j * 6 + j] = d;
         for (int i = j + 1; i < 6; ++i) {
             double t = A[i * 6 + j];
             for (int k = 0; k < j; ++k) t -= L[i * 6 + k] * L[j * 6 + k];
-            L[i * 6 + j] = t / d;
+            L[i * 6 + j] = t * inv;
         }
     }
     return true;
@@ -145,9 +191,9 @@ __device__ double dampingFP64(const double* A, int level) {
     return base;
 }
 
-// Nearest rotation U V^T by one-sided Jacobi SVD in FP64 (projectToSO3,
-// ICPShared.h:309-317), negating U's smallest-singular-value column if det < 0.
-__device__ void projectSO3(const double* M, double* R) {
+// Nearest rotation by one-sided Jacobi SVD in FP64, U V^T with U's smallest-singular-
+// value column negated if det < 0 (projectToSO3, ICPShared.h:309-317). General fallback.
+__device__ void projectSO3Svd(const double* M, double* R) {
     double U[9], V[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
     for (int i = 0; i < 9; ++i) U[i] = M[i];
     for (int sweep = 0; sweep < 60; ++sweep) {
@@ -204,6 +250,36 @@ __device__ void projectSO3(const double* M, double* R) {
             if (sig[j] < sig[js]) js = j;
         for (int k = 0; k < 3; ++k) U[k * 3 + js] = -U[k * 3 + js];
     }
+}
+
+// The same projection for the matrices the update produces (a rotation times a
+// rotation, orthogonal up to float rounding): the polar factor by Newton's iteration
+// X <- (X + X^-T) / 2 (Higham, Functions of Matrices, 8.3), which converges
+// quadratically; it equals U V^T when det > 0. Anything else takes the SVD route.
+__device__ void projectSO3(const double* M, double* R) {
+    double X[9];
+    for (int i = 0; i < 9; ++i) X[i] = M[i];
+    for (int it = 0; it < 8; ++it) {
+        const double c00 = X[4] * X[8] - X[5] * X[7], c01 = X[5] * X[6] - X[3] * X[8], c02 = X[3] * X[7] - X[4] * X[6];
+        const double c10 = X[2] * X[7] - X[1] * X[8], c11 = X[0] * X[8] - X[2] * X[6], c12 = X[1] * X[6] - X[0] * X[7];
+        const double c20 = X[1] * X[5] - X[2] * X[4], c21 = X[2] * X[3] - X[0] * X[5], c22 = X[0] * X[4] - X[1] * X[3];
+        const double det = X[0] * c00 + X[1] * c01 + X[2] * c02;
+        if (!(det > 0.5) || !isfinite(det)) {
+            projectSO3Svd(M, R);
+            return;
+        }
+        const double inv = 1.0 / det;  // X^-T = cofactor(X) / det
+        const double Y[9] = {c00 * inv, c01 * inv, c02 * inv, c10 * inv, c11 * inv, c12 * inv,
+                             c20 * inv, c21 * inv, c22 * inv};
+        double dmax = 0.0;
+        for (int i = 0; i < 9; ++i) {
+            const double nx = 0.5 * (X[i] + Y[i]);
+            dmax = fmax(dmax, fabs(nx - X[i]));
+            X[i] = nx;
+        }
+        if (dmax < 1e-15) break;
+    }
+    for (int i = 0; i < 9; ++i) R[i] = X[i];
 }
 
 // One solve + pose update (trackLevel's loop body, ICPTracker.cpp:79-130), lane 0 of
@@ -313,8 +389,8 @@ __device__ void icpSolve(IcpState* st, const double* sums, int level, int iter) 
         st->pose[12 + i] = nt[i];
     }
     st->final_step = step_norm;
-    st->iterations += 1 << (8 * level);
     st->last_final_step = step_norm;
+    st->iterations += 1 << (8 * level);
     st->error = (float)(residual / (double)(inliers > 1 ? inliers : 1));
     st->inliers = inliers;
     if (step_norm < 5e-5f) {
@@ -325,9 +401,15 @@ __device__ void icpSolve(IcpState* st, const double* sums, int level, int iter) 
 
 __global__ void icpInit(IcpState* st, PoseArg est, PoseArg ref) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    // Both poses are read before any write: either may point into the state itself.
+    float e[16], r[16];
     for (int i = 0; i < 16; ++i) {
-        st->pose[i] = poseEl(est, i);
-        st->ref[i] = poseEl(ref, i);
+        e[i] = poseEl(est, i);
+        r[i] = poseEl(ref, i);
+    }
+    for (int i = 0; i < 16; ++i) {
+        st->pose[i] = e[i];
+        st->ref[i] = r[i];
     }
     for (int i = 0; i < 36; ++i) st->information[i] = 0.0f;
     st->error = 0.0f;
@@ -340,16 +422,122 @@ __global__ void icpInit(IcpState* st, PoseArg est, PoseArg ref) {
     st->iterations = 0;
 }
 
+__global__ void icpClearTicks(IcpState* st) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) st->ticks_sum = st->ticks_solve = st->ticks_n = 0;
+}
+
+// A * B for rigid 4x4 column-major poses, each entry summed left to right
+// ((a0 b0 + a1 b1) + a2 b2) + a3 b3 with the implicit bottom row (0, 0, 0, 1).
+__device__ void composeRigid(const float* A, const float* B, float* C) {
+    for (int c = 0; c < 4; ++c)
+        for (int r = 0; r < 3; ++r) {
+            const float s = ((A[0 * 4 + r] * B[c * 4 + 0] + A[1 * 4 + r] * B[c * 4 + 1]) + A[2 * 4 + r] * B[c * 4 + 2]) +
+                            A[3 * 4 + r] * (c == 3 ? 1.0f : 0.0f);
+            C[c * 4 + r] = s;
+        }
+    C[3] = C[7] = C[11] = 0.0f;
+    C[15] = 1.0f;
+}
+
+// Constant-velocity motion model of the reference pipeline
+// (src/app/PipelineController.cpp:1162-1187): predicted = cur * (last^-1 * cur), then
+// last = cur. The inverse is the rigid one.
+__global__ void predictPose(const float* __restrict__ cur, float* __restrict__ last, float* __restrict__ pred) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    float c[16], l[16], li[16], rel[16], p[16];
+    for (int i = 0; i < 16; ++i) {
+        c[i] = cur[i];
+        l[i] = last[i];
+    }
+    const Rigid L = rigidFromPtr(l);
+    const Rigid Li = rigidInverse(L);
+    for (int r = 0; r < 3; ++r) {
+        for (int k = 0; k < 3; ++k) li[k * 4 + r] = Li.R.m[r][k];
+        li[12 + r] = r == 0 ? Li.t.x : (r == 1 ? Li.t.y : Li.t.z);
+    }
+    li[3] = li[7] = li[11] = 0.0f;
+    li[15] = 1.0f;
+    composeRigid(li, c, rel);
+    composeRigid(c, rel, p);
+    for (int i = 0; i < 16; ++i) {
+        pred[i] = p[i];
+        last[i] = c[i];
+    }
+}
+
 struct PoseDerived {
     M33 R_cw, R_rel;
     F3 t_cw, t_rel;
 };
+
+// The per-pixel part of buildLinearSystem (ICPTracker.cpp:236-327): accumulates one
+// live pixel's terms into (hi, lo) and its gate outcome into cnt.
+__device__ __forceinline__ void icpPixel(const IcpIterArgs& a, const PoseDerived& pd, int x, int y,
+                                         float* hi, float* lo, int* cnt) {
+    const int idx = y * a.W + x;
+    const F3 lv{a.lv[3 * idx], a.lv[3 * idx + 1], a.lv[3 * idx + 2]};
+    if (lv.z <= 0.001f || !f3finite(lv)) return;
+    const F3 vr = f3add(mulv(pd.R_rel, lv), pd.t_rel);
+    if (vr.z <= 0.001f) return;
+    cnt[1]++;
+    const float inv_z = 1.0f / vr.z;
+    const float mxf = a.Km.fx * vr.x * inv_z + a.Km.cx + 0.5f;
+    const float myf = a.Km.fy * vr.y * inv_z + a.Km.cy + 0.5f;
+    // floorToInt (CoordinateMath.h:27-49): finite and representable, then the bounds.
+    if (!isfinite(mxf) || !isfinite(myf)) return;
+    const float fmx = floorf(mxf), fmy = floorf(myf);
+    if (!(fmx >= 0.0f && fmx < (float)a.MW && fmy >= 0.0f && fmy < (float)a.MH)) return;
+    cnt[3]++;
+    const int midx = (int)fmy * a.MW + (int)fmx;
+    const F3 mv{a.mv[3 * midx], a.mv[3 * midx + 1], a.mv[3 * midx + 2]};
+    const F3 mn{a.mn[3 * midx], a.mn[3 * midx + 1], a.mn[3 * midx + 2]};
+    if (!(f3finite(mv) && f3dot(mv, mv) > 1e-12f) || !(f3finite(mn) && f3dot(mn, mn) > 0.9f)) return;
+    cnt[2]++;
+    const F3 vlw = f3add(mulv(pd.R_cw, lv), pd.t_cw);
+    const F3 dd = f3sub(vlw, mv);
+    if (f3dot(dd, dd) > a.dist_sq) {
+        cnt[4]++;
+        return;
+    }
+    const F3 ln{a.ln[3 * idx], a.ln[3 * idx + 1], a.ln[3 * idx + 2]};
+    if (!(f3finite(ln) && f3dot(ln, ln) > 0.9f)) return;
+    const F3 nlw = mulv(pd.R_cw, ln);
+    if (fabsf(f3dot(nlw, mn)) < a.angle_cos) {
+        cnt[5]++;
+        return;
+    }
+    const float err = f3dot(mn, dd);
+    if (!isfinite(err)) return;
+    const F3 nlc = mulvT(pd.R_cw, mn);
+    const F3 cr = f3cross(lv, nlc);
+    const float J[6] = {nlc.x, nlc.y, nlc.z, cr.x, cr.y, cr.z};
+    if (!(isfinite(J[0]) && isfinite(J[1]) && isfinite(J[2]) && isfinite(J[3]) && isfinite(J[4]) && isfinite(J[5])))
+        return;
+    const float ae = fabsf(err);
+    const float w = (ae <= 0.02f) ? 1.0f : 0.02f / ae;
+    const float we = err * w;
+    const float loss = (ae <= 0.02f) ? ae * ae : 2.0f * 0.02f * ae - 0.02f * 0.02f;
+    int k = 0;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+#pragma unroll
+        for (int j = i; j < 6; ++j) {
+            twoSum(hi[k], lo[k], w * J[i] * J[j], 0.0f);
+            ++k;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 6; ++i) twoSum(hi[21 + i], lo[21 + i], -(J[i] * we), 0.0f);
+    twoSum(hi[27], lo[27], loss, 0.0f);
+    cnt[0]++;
+}
 
 __global__ __launch_bounds__(256) void icpIter(IcpIterArgs a, IcpState* __restrict__ st,
                                                double* __restrict__ partials, unsigned* __restrict__ ticket) {
     __shared__ PoseDerived sp;
     __shared__ float s_hi[4][kIcpSlotsF], s_lo[4][kIcpSlotsF];
     __shared__ int s_cnt[4][kIcpSlotsI];
+    __shared__ double s_grp[kSumGroups][kIcpSlots];
     __shared__ double s_sum[kIcpSlots];
     __shared__ bool s_last;
     if (st->stop_level == a.level) return;  // uniform over the grid
@@ -364,8 +552,7 @@ __global__ __launch_bounds__(256) void icpIter(IcpIterArgs a, IcpState* __restri
         sp.t_rel = f3add(mulv(rinv.R, pose.t), rinv.t);
     }
     __syncthreads();
-    const M33 R_cw = sp.R_cw, R_rel = sp.R_rel;
-    const F3 t_cw = sp.t_cw, t_rel = sp.t_rel;
+    const PoseDerived pd = sp;
 
     float hi[kIcpSlotsF], lo[kIcpSlotsF];
 #pragma unroll
@@ -376,77 +563,15 @@ __global__ __launch_bounds__(256) void icpIter(IcpIterArgs a, IcpState* __restri
     int cnt[kIcpSlotsI] = {0, 0, 0, 0, 0, 0};
     const int iw = a.W - 2, ih = a.H - 2;
     const int npx = (iw > 0 && ih > 0) ? iw * ih : 0;
-    for (int p = blockIdx.x * 256 + tid; p < npx; p += gridDim.x * 256) {
-        const int x = 1 + p % iw, y = 1 + p / iw;
-        const int idx = y * a.W + x;
-        const F3 lv{a.lv[3 * idx], a.lv[3 * idx + 1], a.lv[3 * idx + 2]};
-        if (lv.z <= 0.001f || !f3finite(lv)) continue;
-        const F3 vr = f3add(mulv(R_rel, lv), t_rel);
-        if (vr.z <= 0.001f) continue;
-        cnt[1]++;
-        const float inv_z = 1.0f / vr.z;
-        const float mxf = a.Km.fx * vr.x * inv_z + a.Km.cx + 0.5f;
-        const float myf = a.Km.fy * vr.y * inv_z + a.Km.cy + 0.5f;
-        // floorToInt: finite and representable as int, then the image bounds.
-        if (!isfinite(mxf) || !isfinite(myf)) continue;
-        const float fmx = floorf(mxf), fmy = floorf(myf);
-        if (!(fmx >= 0.0f && fmx < (float)a.MW && fmy >= 0.0f && fmy < (float)a.MH)) continue;
-        cnt[3]++;
-        const int midx = (int)fmy * a.MW + (int)fmx;
-        const F3 mv{a.mv[3 * midx], a.mv[3 * midx + 1], a.mv[3 * midx + 2]};
-        const F3 mn{a.mn[3 * midx], a.mn[3 * midx + 1], a.mn[3 * midx + 2]};
-        if (!(f3finite(mv) && f3dot(mv, mv) > 1e-12f) || !(f3finite(mn) && f3dot(mn, mn) > 0.9f)) continue;
-        cnt[2]++;
-        const F3 vlw = f3add(mulv(R_cw, lv), t_cw);
-        const F3 dd = f3sub(vlw, mv);
-        if (f3dot(dd, dd) > a.dist_sq) {
-            cnt[4]++;
-            continue;
-        }
-        const F3 ln{a.ln[3 * idx], a.ln[3 * idx + 1], a.ln[3 * idx + 2]};
-        if (!(f3finite(ln) && f3dot(ln, ln) > 0.9f)) continue;
-        const F3 nlw = mulv(R_cw, ln);
-        if (fabsf(f3dot(nlw, mn)) < a.angle_cos) {
-            cnt[5]++;
-            continue;
-        }
-        const float err = f3dot(mn, dd);
-        if (!isfinite(err)) continue;
-        const F3 nlc = mulvT(R_cw, mn);
-        const F3 cr = f3cross(lv, nlc);
-        const float J[6] = {nlc.x, nlc.y, nlc.z, cr.x, cr.y, cr.z};
-        if (!(isfinite(J[0]) && isfinite(J[1]) && isfinite(J[2]) && isfinite(J[3]) && isfinite(J[4]) && isfinite(J[5])))
-            continue;
-        const float ae = fabsf(err);
-        const float w = (ae <= 0.02f) ? 1.0f : 0.02f / ae;
-        const float we = err * w;
-        const float loss = (ae <= 0.02f) ? ae * ae : 2.0f * 0.02f * ae - 0.02f * 0.02f;
-        int k = 0;
+    for (int p = blockIdx.x * 256 + tid; p < npx; p += gridDim.x * 256)
+        icpPixel(a, pd, 1 + p % iw, 1 + p / iw, hi, lo, cnt);
+
+    // Wave fold to lane 63 (compensated pairs, exact integers), then across waves.
 #pragma unroll
-        for (int i = 0; i < 6; ++i) {
+    for (int i = 0; i < kIcpSlotsF; ++i) waveSumPair(hi[i], lo[i]);
 #pragma unroll
-            for (int j = i; j < 6; ++j) {
-                twoSum(hi[k], lo[k], w * J[i] * J[j], 0.0f);
-                ++k;
-            }
-        }
-#pragma unroll
-        for (int i = 0; i < 6; ++i) twoSum(hi[21 + i], lo[21 + i], -(J[i] * we), 0.0f);
-        twoSum(hi[27], lo[27], loss, 0.0f);
-        cnt[0]++;
-    }
-    // Wave fold (compensated pairs, exact integers).
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-#pragma unroll
-        for (int i = 0; i < kIcpSlotsF; ++i) {
-            const float oh = __shfl_xor(hi[i], o), ol = __shfl_xor(lo[i], o);
-            twoSum(hi[i], lo[i], oh, ol);
-        }
-#pragma unroll
-        for (int i = 0; i < kIcpSlotsI; ++i) cnt[i] += __shfl_xor(cnt[i], o);
-    }
-    if (lane == 0) {
+    for (int i = 0; i < kIcpSlotsI; ++i) cnt[i] = waveSumInt(cnt[i]);
+    if (lane == 63) {
 #pragma unroll
         for (int i = 0; i < kIcpSlotsF; ++i) {
             s_hi[wv][i] = hi[i];
@@ -467,7 +592,7 @@ __global__ __launch_bounds__(256) void icpIter(IcpIterArgs a, IcpState* __restri
             const int i = tid - kIcpSlotsF;
             v = (double)(s_cnt[0][i] + s_cnt[1][i] + s_cnt[2][i] + s_cnt[3][i]);
         }
-        partials[(size_t)tid * nb + blockIdx.x] = v;
+        partials[(size_t)tid * kIcpMaxBlocks + blockIdx.x] = v;
     }
     __threadfence();
     __syncthreads();
@@ -477,17 +602,41 @@ __global__ __launch_bounds__(256) void icpIter(IcpIterArgs a, IcpState* __restri
     }
     __syncthreads();
     if (!s_last) return;
+    const unsigned long long t_last = __builtin_amdgcn_s_memrealtime();
     __threadfence();
-    // Last block: FP64 cross-block sum in block order, one wave per slot group.
-    for (int slot = wv; slot < kIcpSlots; slot += 4) {
-        double s = 0.0;
-        for (unsigned bi = lane; bi < nb; bi += 64) s += partials[(size_t)slot * nb + bi];
+    // Last block: FP64 cross-block sum. Thread (g, s) sums slot s over blocks g, g + 7,
+    // g + 14, ... with all of its loads issued before the first add; the seven group
+    // sums are then added in group order. The order is fixed, so the result is too.
+    if (tid < kSumGroups * kIcpSlots) {
+        const int s = tid % kIcpSlots, g = tid / kIcpSlots;
+        const double* src = partials + (size_t)s * kIcpMaxBlocks;
+        constexpr int kMaxPer = (kIcpMaxBlocks + kSumGroups - 1) / kSumGroups;
+        double v[kMaxPer];
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-        if (lane == 0) s_sum[slot] = s;
+        for (int i = 0; i < kMaxPer; ++i) {
+            const unsigned bi = (unsigned)(g + kSumGroups * i);
+            v[i] = bi < nb ? src[bi] : 0.0;
+        }
+        double acc = 0.0;
+#pragma unroll
+        for (int i = 0; i < kMaxPer; ++i) acc += v[i];
+        s_grp[g][s] = acc;
     }
     __syncthreads();
-    if (tid == 0) icpSolve(st, s_sum, a.level, a.iter);
+    if (tid < kIcpSlots) {
+        double acc = 0.0;
+        for (int g = 0; g < kSumGroups; ++g) acc += s_grp[g][tid];
+        s_sum[tid] = acc;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const unsigned long long t_sum = __builtin_amdgcn_s_memrealtime();
+        icpSolve(st, s_sum, a.level, a.iter);
+        const unsigned long long t_end = __builtin_amdgcn_s_memrealtime();
+        st->ticks_sum += t_sum - t_last;
+        st->ticks_solve += t_end - t_sum;
+        st->ticks_n += 1;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -496,15 +645,20 @@ __global__ __launch_bounds__(256) void icpIter(IcpIterArgs a, IcpState* __restri
 size_t icpStateBytes() { return sizeof(IcpState); }
 int icpBlocksFor(int W, int H) {
     const long long npx = (long long)(W > 2 ? W - 2 : 0) * (H > 2 ? H - 2 : 0);
-    long long b = (npx + 511) / 512;
+    long long b = (npx + 255) / 256;
     if (b < 1) b = 1;
-    if (b > 1024) b = 1024;
+    if (b > kIcpMaxBlocks) b = kIcpMaxBlocks;
     return (int)b;
 }
-size_t icpPartialsBytes(int maxBlocks) { return sizeof(double) * kIcpSlots * (size_t)maxBlocks; }
+size_t icpPartialsBytes(int) { return sizeof(double) * kIcpSlots * (size_t)kIcpMaxBlocks; }
 
 hipError_t launchIcpInit(void* st, const PoseArg& est, const PoseArg& ref, hipStream_t s) {
     hipLaunchKernelGGL(icpInit, dim3(1), dim3(64), 0, s, (IcpState*)st, est, ref);
+    return hipGetLastError();
+}
+
+hipError_t launchPredict(const float* cur, float* last, float* pred, hipStream_t s) {
+    hipLaunchKernelGGL(predictPose, dim3(1), dim3(64), 0, s, cur, last, pred);
     return hipGetLastError();
 }
 
@@ -527,6 +681,18 @@ hipError_t launchIcpIter(const float* lv, const float* ln, int W, int H, const f
     a.iter = iter;
     hipLaunchKernelGGL(icpIter, dim3(icpBlocksFor(W, H)), dim3(256), 0, s, a, (IcpState*)st, partials, ticket);
     return hipGetLastError();
+}
+
+hipError_t launchIcpClearTicks(void* st, hipStream_t s) {
+    hipLaunchKernelGGL(icpClearTicks, dim3(1), dim3(64), 0, s, (IcpState*)st);
+    return hipGetLastError();
+}
+
+void icpTicksRead(const void* hostCopy, unsigned long long* ticks) {
+    const IcpState* s = static_cast<const IcpState*>(hostCopy);
+    ticks[0] = s->ticks_sum;
+    ticks[1] = s->ticks_solve;
+    ticks[2] = s->ticks_n;
 }
 
 // Host view of the state (copied back once per frame).

@@ -101,6 +101,9 @@ _SIGS = {
                                    c_void_p]),
     "azu_raycast_dpose": (ctypes.c_int, [c_void_p, c_void_p, c_float_p, ctypes.c_int, ctypes.c_int, c_void_p,
                                          c_void_p, c_void_p, c_void_p]),
+    "azu_raycast_depth_loss": (ctypes.c_int, [c_void_p, c_float_p, c_float_p, ctypes.c_int, ctypes.c_int, c_void_p,
+                                              ctypes.c_float, ctypes.c_int, c_void_p, c_void_p,
+                                              ctypes.POINTER(ctypes.c_double), c_void_p]),
     "azu_tracker_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, c_float_p, ctypes.POINTER(IcpParams),
                                           ctypes.POINTER(c_void_p)]),
     "azu_tracker_destroy": (ctypes.c_int, [c_void_p]),
@@ -117,8 +120,10 @@ _SIGS = {
     "azu_tracker_track_dpose": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_int, ctypes.c_int, c_void_p,
                                                c_void_p, c_void_p]),
     "azu_tracker_set_pose": (ctypes.c_int, [c_void_p, c_float_p, c_void_p]),
+    "azu_tracker_predict": (ctypes.c_int, [c_void_p, ctypes.POINTER(c_void_p), c_void_p]),
     "azu_tracker_pose_device": (ctypes.c_int, [c_void_p, ctypes.POINTER(c_void_p)]),
     "azu_tracker_result": (ctypes.c_int, [c_void_p, ctypes.POINTER(IcpResult), c_void_p]),
+    "azu_tracker_phase_times": (ctypes.c_int, [c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_int]),
     "azu_profile_enable": (ctypes.c_int, [ctypes.c_int]),
     "azu_profile_reset": (ctypes.c_int, []),
     "azu_profile_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong),

@@ -3,8 +3,10 @@
 
   frame 0:   live pyramid; integrate at the initial pose; raycast the model there;
   frame k:   live pyramid; ICP against the model raycast at the previous tracked pose
-             (estimate = reference pose = that pose); integrate and raycast at the
-             tracked pose.
+             (reference pose = estimate = that pose); integrate and raycast at the
+             tracked pose. The reference pipeline's constant-velocity estimate
+             (PipelineController.cpp:1162-1187) is available as motion_model=
+             "constant_velocity"; the benchmark loop does not use it.
 
 The tracked pose never leaves the device inside the loop (the ICP state feeds
 integrate and raycast through a device pointer); each frame's pose is copied to a
@@ -21,7 +23,10 @@ from .native import check, lib
 
 
 class FrameLoop:
-    def __init__(self, tsdf_params, fmt, W, H, K, icp_params=None, mode=N.MODE_AUTO, stream=None, max_frames=4096):
+    def __init__(self, tsdf_params, fmt, W, H, K, icp_params=None, mode=N.MODE_AUTO, stream=None, max_frames=4096,
+                 motion_model="none"):
+        """motion_model: "none" (the estimate is the previous tracked pose, the benchmark
+        loop) or "constant_velocity" (the reference pipeline's cur * (last^-1 * cur))."""
         self.vol = TSDFVolume(tsdf_params if isinstance(tsdf_params, TSDFParams) else TSDFParams(**tsdf_params),
                               fmt=fmt, mode=mode)
         self.icp = icp_params or ICPParams(min_depth=self.vol.params.min_depth, max_depth=self.vol.params.max_depth)
@@ -38,6 +43,7 @@ class FrameLoop:
         check(lib.azu_host_alloc(ctypes.byref(p), 64 * max_frames))
         self._poses_host = p.value
         self.nframes = 0
+        self.motion_model = motion_model
 
     def reset(self, pose0):
         self.vol.reset(self.stream)
@@ -49,7 +55,13 @@ class FrameLoop:
         s = self.stream
         check(lib.azu_tracker_prepare_mm(self.tr.h, d_mm, s))
         if self.nframes > 0:
-            check(lib.azu_tracker_track_dpose(self.tr.h, self.mv.ptr, self.mn.ptr, self.W, self.H, self.d_pose,
+            # reference pose: the model's raycast pose (the previous tracked pose)
+            est = self.d_pose
+            if self.motion_model == "constant_velocity":
+                pred = ctypes.c_void_p()
+                check(lib.azu_tracker_predict(self.tr.h, ctypes.byref(pred), s))
+                est = pred.value
+            check(lib.azu_tracker_track_dpose(self.tr.h, self.mv.ptr, self.mn.ptr, self.W, self.H, est,
                                               self.d_pose, s))
         lo, hi = self.vol.params.min_depth, self.vol.params.max_depth
         check(lib.azu_integrate_dpose(self.vol.h, self.d0, self.W, self.H, N.fptr(self.K), self.d_pose, lo, hi, s))

@@ -146,6 +146,20 @@ int azu_raycast(azu_volume* v, const float pose[16], const float K[4], int W, in
 int azu_raycast_dpose(azu_volume* v, const float* d_pose, const float K[4], int W, int H, float* d_vertices,
                       float* d_normals, float* d_t_hit, azu_stream s);
 
+/* Raycast-depth loss and its adjoint (no counterpart in the reference; march and secant
+ * as TSDFVolume::raycast, TSDFVolume.cpp:384-445, with a uniform step of step_voxels
+ * voxels on a lattice anchored at min_depth |ray|):
+ *   L = sum_pixels (d_ray - d_obs)^2 over pixels with a hit and d_obs > 0 (metres).
+ * out[0] = L, out[1..6] = dL/dxi for the pose perturbation pose <- pose * [R(w) | t],
+ * xi = [t; w] (the ICP increment convention, ICPTracker.cpp:100-121), out[7] = the
+ * number of contributing pixels. backward == 0 computes L only. d_grad_tsdf (res^3
+ * floats, may be NULL) receives += dL/dtsdf (float atomics: summation order not fixed).
+ * d_depth_out (W*H, may be NULL) receives the ray depths (0 on a miss). step_voxels must
+ * be in [0.05, 8]. Synchronises. */
+int azu_raycast_depth_loss(azu_volume* v, const float pose[16], const float K[4], int W, int H,
+                           const float* d_depth_obs, float step_voxels, int backward, float* d_grad_tsdf,
+                           float* d_depth_out, double out[8], azu_stream s);
+
 /* ---- ICP tracker (kfusion::tracking::ICPTracker, include/tracking/ICPTracker.h:87-134) ---- */
 /* W x H is the live frame size; K its intrinsics (also the camera of the model image). */
 int azu_tracker_create(int W, int H, const float K[4], const azu_icp_params* p, azu_tracker** out);
@@ -177,10 +191,19 @@ int azu_tracker_track_dpose(azu_tracker* t, const float* d_model_v, const float*
 /* Sets the tracker's device pose (the one azu_tracker_pose_device points at) without
  * tracking, e.g. the first frame's pose before any model exists. */
 int azu_tracker_set_pose(azu_tracker* t, const float pose[16], azu_stream s);
+/* The reference pipeline's constant-velocity estimate (src/app/PipelineController.cpp:
+ * 1162-1187): predicted = current * (last^-1 * current), then last = current, where
+ * current is the tracker's device pose. *d_predicted receives the device pointer of the
+ * prediction (16 floats), to pass as the estimate of azu_tracker_track_dpose. */
+int azu_tracker_predict(azu_tracker* t, const float** d_predicted, azu_stream s);
 /* Device pointer to the tracked pose (16 floats), valid once the enqueued track ran. */
 int azu_tracker_pose_device(azu_tracker* t, const float** d_pose);
 /* Synchronises the stream and returns the result of the last track. */
 int azu_tracker_result(azu_tracker* t, azu_icp_result* out, azu_stream s);
+/* Diagnostic: device clock time (ms) of the ICP iterations' last-block phases summed
+ * since the last reset: out[0] cross-block FP64 sum, out[1] solve + pose update,
+ * out[2] the number of iterations that reached them. Synchronises; reset != 0 clears. */
+int azu_tracker_phase_times(azu_tracker* t, double out[3], int reset);
 
 /* ---- per-kernel timing (HIP events on the launch stream) ---- */
 /* While enabled, every kernel launch is bracketed by events; azu_profile_read

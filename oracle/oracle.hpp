@@ -107,6 +107,28 @@ template <class T> inline Pose<T> inverse(const Pose<T>& P) {
     return I;
 }
 
+// A * B for rigid poses, each entry summed left to right ((a0 b0 + a1 b1) + a2 b2) + a3 b3
+// (Eigen's 4x4 column-packet order) with the implicit bottom row (0, 0, 0, 1).
+template <class T> inline Pose<T> compose(const Pose<T>& A, const Pose<T>& B) {
+    Pose<T> C;
+    const T at[3] = {A.t.x, A.t.y, A.t.z}, bt[3] = {B.t.x, B.t.y, B.t.z};
+    T ct[3];
+    for (int r = 0; r < 3; ++r) {
+        for (int c = 0; c < 3; ++c)
+            C.R.m[r][c] = ((A.R.m[r][0] * B.R.m[0][c] + A.R.m[r][1] * B.R.m[1][c]) + A.R.m[r][2] * B.R.m[2][c]) +
+                          at[r] * T(0);
+        ct[r] = ((A.R.m[r][0] * bt[0] + A.R.m[r][1] * bt[1]) + A.R.m[r][2] * bt[2]) + at[r] * T(1);
+    }
+    C.t = {ct[0], ct[1], ct[2]};
+    return C;
+}
+
+// Constant-velocity estimate of the reference pipeline (src/app/PipelineController.cpp:
+// 1162-1187): cur * (last^-1 * cur).
+template <class T> inline Pose<T> predictPose(const Pose<T>& cur, const Pose<T>& last) {
+    return compose(cur, compose(inverse(last), cur));
+}
+
 struct Intr {
     float fx, fy, cx, cy;
 };

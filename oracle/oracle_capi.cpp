@@ -170,14 +170,15 @@ template <class T> struct Pyr {
     }                                                                                                        \
     /* The benchmark frame loop over a uint16 mm sequence (the pipeline order): frame 0 integrates at the    \
        given pose and raycasts; frame k >= 1 builds the live pyramid, tracks against the model raycast at    \
-       the previous tracked pose (ref = that pose, estimate = that pose), integrates and raycasts at the     \
-       tracked pose. poses_out: 16 floats per frame. ms_out: 5 doubles per frame {pyramid, icp, integrate,   \
+       the previous tracked pose (ref = that pose), integrates                                              \
+       and raycasts at the tracked pose (velocity != 0: the estimate is the constant-velocity prediction,  \
+       else the previous pose). poses_out: 16 floats per frame. ms_out: 5 doubles per frame {pyramid, icp, integrate,   \
        raycast, total}. When poses_in is given, frames k >= 1 use poses_in[k] instead of ICP (integrate +    \
        raycast only). Returns the number of frames run. */                                                   \
     ORC_API int orc_run_sequence_##SFX(void* h, const uint16_t* frames_mm, int nframes, int W, int H,        \
                                        const float* K, const float* pose0, const int* iters, float dist,     \
                                        float angle, const float* poses_in, float* poses_out, double* ms_out,  \
-                                       long long* n_upd_out) {                                               \
+                                       long long* n_upd_out, int velocity) {                                 \
         auto& V = *static_cast<Volume<T>*>(h);                                                               \
         const IcpParams prm = mkIcp(iters, dist, angle);                                                     \
         const Intr KK = mkK(K);                                                                              \
@@ -185,6 +186,7 @@ template <class T> struct Pyr {
         std::vector<T> depth(npx);                                                                           \
         std::vector<V3<T>> mv(npx), mn(npx);                                                                 \
         Pose<T> pose = poseFrom<T>(pose0);                                                                   \
+        Pose<T> last = pose;                                                                                 \
         using clk = std::chrono::steady_clock;                                                               \
         auto ms = [](clk::time_point a, clk::time_point b) {                                                 \
             return std::chrono::duration<double, std::milli>(b - a).count();                                \
@@ -204,7 +206,9 @@ template <class T> struct Pyr {
                     Level<T> L[3];                                                                           \
                     P.levels(L);                                                                             \
                     t1 = clk::now();                                                                         \
-                    const IcpResult<T> r = track(L, mv.data(), mn.data(), W, H, KK, pose, pose, prm);        \
+                    const Pose<T> est = velocity ? predictPose(pose, last) : pose;                           \
+                    last = pose;                                                                             \
+                    const IcpResult<T> r = track(L, mv.data(), mn.data(), W, H, KK, est, pose, prm);         \
                     pose = poseFrom<T>(r.pose);                                                              \
                     t2 = clk::now();                                                                         \
                 }                                                                                            \

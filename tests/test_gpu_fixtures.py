@@ -55,7 +55,8 @@ def test_pyramid_jump_guard_gpu():
                                       (N.FMT_F16_U16, N.MODE_EXACT), (N.FMT_F16_U16, N.MODE_FAST)])
 def test_integrate_hand_fold_gpu(fmt, mode):
     """tests/tsdf_integration_race_contract.cpp:232-324 for every storage form and kernel.
-    FP16 tsdf storage rounds to 11 significant bits, so its bar is 2.5e-4 instead of 2e-5."""
+    FP16 tsdf storage rounds to 11 significant bits after every fold; two folds of values
+    in [0.5, 1) can drift by one binary16 ulp there (2^-11), so its bar is 5e-4, not 2e-5."""
     g = TSDFVolume(TSDFParams(**fx.FOLD_PARAMS), fmt=fmt, mode=mode)
     I4 = np.eye(4, dtype=np.float32)
     for d in fx.fold_frames():
@@ -66,5 +67,5 @@ def test_integrate_hand_fold_gpu(fmt, mode):
     else:
         for k in range(1, 64):
             te, we = fx.fold_expect(k)
-            assert w[k, 32, 32] == we and abs(float(t[k, 32, 32]) - te) < 2.5e-4
+            assert w[k, 32, 32] == we and abs(float(t[k, 32, 32]) - te) < 5e-4
         assert w[50, 32, 32] == 2.0 and w[54, 32, 32] == 0.0 and w[50, 32, 33] == 0.0
