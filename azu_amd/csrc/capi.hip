@@ -32,6 +32,15 @@ hipError_t launchIcpInit(void* st, const PoseArg& est, const PoseArg& ref, hipSt
 hipError_t launchPredict(const float* cur, float* last, float* pred, hipStream_t s);
 hipError_t launchIcpClearTicks(void* st, hipStream_t s);
 int depthLossBlocks(int W, int H);
+struct McResult {
+    unsigned long long ntri, nvert;
+    bool truncated;
+    float* pos;
+    float* nrm;
+    uint32_t* idx;
+};
+hipError_t launchMarchingCubes(int format, const void* vox, const VolParams& p, unsigned long long max_tris,
+                               McResult* res, hipStream_t st);
 hipError_t launchDepthLoss(int format, const void* vox, const VolParams& p, const PoseArg& pa, Intr K, int W, int H,
                            const float* d_obs, float step, int backward, float* grad_tsdf, float* depth_out,
                            double* partials, double* out8, hipStream_t st);
@@ -191,7 +200,7 @@ struct azu_volume {
     size_t adj_bytes = 0;
     double* adj_out = nullptr;               // depth-loss result [8]
     int frames = 0;
-    bool exact() const { return mode == AZU_MODE_EXACT || (mode == AZU_MODE_AUTO && format == AZU_FMT_F32); }
+    bool exact() const { return mode != AZU_MODE_FAST; }
 };
 
 struct azu_tracker {
@@ -543,6 +552,64 @@ int azu_raycast_depth_loss(azu_volume* v, const float pose[16], const float K[4]
     }
     AZU_HIP_TRY(hipMemcpyAsync(out, v->adj_out, 8 * sizeof(double), hipMemcpyDeviceToHost, st), "loss read");
     AZU_HIP_TRY(hipStreamSynchronize(st), "loss sync");
+    return AZU_OK;
+}
+
+// ---- marching cubes --------------------------------------------------------------
+struct azu_mesh {
+    McResult r;
+};
+
+int azu_mc_extract(azu_volume* v, unsigned long long max_triangles, azu_mesh** out, azu_stream s) {
+    if (!v || !out) return fail(AZU_ERR_ARG, "null volume or output");
+    *out = nullptr;
+    auto* m = new azu_mesh();
+    hipError_t e;
+    {
+        ProfScope ps("marching_cubes", (hipStream_t)s);
+        e = launchMarchingCubes(v->format, v->vox, toVol(v->params), max_triangles, &m->r, (hipStream_t)s);
+    }
+    if (e != hipSuccess) {
+        delete m;
+        return hipFail(e, "marching cubes");
+    }
+    *out = m;
+    return AZU_OK;
+}
+
+int azu_mesh_info(const azu_mesh* m, unsigned long long* vertices, unsigned long long* triangles, int* truncated) {
+    if (!m) return fail(AZU_ERR_ARG, "null mesh");
+    if (vertices) *vertices = m->r.nvert;
+    if (triangles) *triangles = m->r.ntri;
+    if (truncated) *truncated = m->r.truncated ? 1 : 0;
+    return AZU_OK;
+}
+
+int azu_mesh_download(const azu_mesh* m, float* positions, float* normals, uint32_t* indices) {
+    if (!m) return fail(AZU_ERR_ARG, "null mesh");
+    if (positions && m->r.nvert)
+        AZU_HIP_TRY(hipMemcpy(positions, m->r.pos, m->r.nvert * 12, hipMemcpyDeviceToHost), "mesh download");
+    if (normals && m->r.nvert)
+        AZU_HIP_TRY(hipMemcpy(normals, m->r.nrm, m->r.nvert * 12, hipMemcpyDeviceToHost), "mesh download");
+    if (indices && m->r.ntri)
+        AZU_HIP_TRY(hipMemcpy(indices, m->r.idx, m->r.ntri * 12, hipMemcpyDeviceToHost), "mesh download");
+    return AZU_OK;
+}
+
+int azu_mesh_device(const azu_mesh* m, const float** p, const float** n, const uint32_t** i) {
+    if (!m) return fail(AZU_ERR_ARG, "null mesh");
+    if (p) *p = m->r.pos;
+    if (n) *n = m->r.nrm;
+    if (i) *i = m->r.idx;
+    return AZU_OK;
+}
+
+int azu_mesh_destroy(azu_mesh* m) {
+    if (!m) return AZU_OK;
+    if (m->r.pos) (void)hipFree(m->r.pos);
+    if (m->r.nrm) (void)hipFree(m->r.nrm);
+    if (m->r.idx) (void)hipFree(m->r.idx);
+    delete m;
     return AZU_OK;
 }
 

@@ -131,6 +131,24 @@ class TSDFVolume:
             return v, n, dt.download(np.float32, (H, W))
         return v, n
 
+    def marching_cubes(self, max_triangles=2000000):
+        """MarchingCubes::extract on the device -> (positions (V, 3), normals (V, 3),
+        indices (3T,) uint32, truncated)."""
+        m = ctypes.c_void_p()
+        check(lib.azu_mc_extract(self.h, max_triangles, ctypes.byref(m), None))
+        try:
+            nv = ctypes.c_ulonglong()
+            nt = ctypes.c_ulonglong()
+            tr = ctypes.c_int()
+            check(lib.azu_mesh_info(m.value, ctypes.byref(nv), ctypes.byref(nt), ctypes.byref(tr)))
+            pos = np.empty((nv.value, 3), np.float32)
+            nrm = np.empty((nv.value, 3), np.float32)
+            idx = np.empty(3 * nt.value, np.uint32)
+            check(lib.azu_mesh_download(m.value, pos.ctypes.data, nrm.ctypes.data, idx.ctypes.data))
+        finally:
+            lib.azu_mesh_destroy(m.value)
+        return pos, nrm, idx, bool(tr.value)
+
     def storage(self):
         p = ctypes.c_void_p()
         b = ctypes.c_size_t()

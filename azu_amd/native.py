@@ -104,6 +104,13 @@ _SIGS = {
     "azu_raycast_depth_loss": (ctypes.c_int, [c_void_p, c_float_p, c_float_p, ctypes.c_int, ctypes.c_int, c_void_p,
                                               ctypes.c_float, ctypes.c_int, c_void_p, c_void_p,
                                               ctypes.POINTER(ctypes.c_double), c_void_p]),
+    "azu_mc_extract": (ctypes.c_int, [c_void_p, ctypes.c_ulonglong, ctypes.POINTER(c_void_p), c_void_p]),
+    "azu_mesh_info": (ctypes.c_int, [c_void_p, ctypes.POINTER(ctypes.c_ulonglong), ctypes.POINTER(ctypes.c_ulonglong),
+                                     ctypes.POINTER(ctypes.c_int)]),
+    "azu_mesh_download": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "azu_mesh_device": (ctypes.c_int, [c_void_p, ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p),
+                                       ctypes.POINTER(c_void_p)]),
+    "azu_mesh_destroy": (ctypes.c_int, [c_void_p]),
     "azu_tracker_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, c_float_p, ctypes.POINTER(IcpParams),
                                           ctypes.POINTER(c_void_p)]),
     "azu_tracker_destroy": (ctypes.c_int, [c_void_p]),

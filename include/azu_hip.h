@@ -43,8 +43,10 @@ typedef enum azu_status {
 typedef enum azu_format { AZU_FMT_F32 = 0, AZU_FMT_F16_U16 = 1 } azu_format;
 
 /* Kernel arithmetic. EXACT: the FP32 parity kernels (IEEE division, no contraction,
- * the reference's incremental x-step); FAST: hardware reciprocals and FMA. AUTO picks
- * EXACT for F32 and FAST for F16_U16. */
+ * the reference's incremental x-step), used for either storage form; FAST: hardware
+ * reciprocals and FMA, about 20% faster integration and raycast, results within
+ * rounding of EXACT but not bit-identical (so an ICP that is not converged within its
+ * iteration budget can land elsewhere). AUTO is EXACT. */
 typedef enum azu_mode { AZU_MODE_AUTO = 0, AZU_MODE_EXACT = 1, AZU_MODE_FAST = 2 } azu_mode;
 
 typedef void* azu_stream; /* hipStream_t */
@@ -159,6 +161,21 @@ int azu_raycast_dpose(azu_volume* v, const float* d_pose, const float K[4], int 
 int azu_raycast_depth_loss(azu_volume* v, const float pose[16], const float K[4], int W, int H,
                            const float* d_depth_obs, float step_voxels, int backward, float* d_grad_tsdf,
                            float* d_depth_out, double out[8], azu_stream s);
+
+/* ---- marching cubes (kfusion::meshing::MarchingCubes, include/meshing/MarchingCubes.h:35-121) ---- */
+typedef struct azu_mesh azu_mesh;
+/* MarchingCubes::extract (src/meshing/MarchingCubes.cpp:174-382) on the device: the same
+ * vertices, normals and triangle indices, in the same order, as the CPU mesher (colour is
+ * not stored by this volume, so no colours are produced). max_triangles is the triangle
+ * cap (MarchingCubes.h:49-62, default 2000000); a capped mesh reports truncated = 1.
+ * Synchronises. */
+int azu_mc_extract(azu_volume* v, unsigned long long max_triangles, azu_mesh** out, azu_stream s);
+int azu_mesh_info(const azu_mesh* m, unsigned long long* vertices, unsigned long long* triangles, int* truncated);
+/* Host copies: positions / normals vertices*3 floats, indices triangles*3. */
+int azu_mesh_download(const azu_mesh* m, float* positions, float* normals, uint32_t* indices);
+int azu_mesh_device(const azu_mesh* m, const float** d_positions, const float** d_normals,
+                    const uint32_t** d_indices);
+int azu_mesh_destroy(azu_mesh* m);
 
 /* ---- ICP tracker (kfusion::tracking::ICPTracker, include/tracking/ICPTracker.h:87-134) ---- */
 /* W x H is the live frame size; K its intrinsics (also the camera of the model image). */

@@ -37,7 +37,7 @@ for sfx, T in (("f32", F), ("f64", D)):
     getattr(lib, "orc_icp_system_" + sfx).argtypes = [P, P, I, I, P, P, I, I, P, P, P, F, F, P, P, P, P]
     getattr(lib, "orc_icp_track_" + sfx).argtypes = [P] * 6 + [I, I, P, P, I, I, P, P, P, P, F, F, P, P, P, P]
     getattr(lib, "orc_run_sequence_" + sfx).restype = I
-    getattr(lib, "orc_run_sequence_" + sfx).argtypes = [P, P, I, I, I, P, P, P, F, F, P, P, P, P, I]
+    getattr(lib, "orc_run_sequence_" + sfx).argtypes = [P, P, I, I, I, P, P, P, F, F, P, P, P, P, I, P]
 lib.orc_jacobi_eigenvalues.argtypes = [P, I, P]
 lib.orc_ldlt_solve_f32.restype = I
 lib.orc_ldlt_solve_f32.argtypes = [P, P, P]
@@ -134,9 +134,11 @@ class Volume:
         pin = None
         if poses_in is not None:
             pin = np.ascontiguousarray(np.stack([cm(T) for T in poses_in]), np.float32)
+        icp = np.zeros((F_, 4), np.float64)  # converged, final_step, inliers, packed steps per level
         self.fn("run_sequence")(self.h, _d(fr), F_, W, H, _d(Kc), _d(cm(pose0)), _d(it), dist, angle,
                                 _d(pin) if pin is not None else None, _d(poses_out), _d(ms), _d(nupd),
-                                1 if velocity else 0)
+                                1 if velocity else 0, _d(icp))
+        self.last_icp = icp
         return np.stack([rm(p) for p in poses_out]), ms, nupd
 
     def close(self):

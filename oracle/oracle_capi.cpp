@@ -178,7 +178,7 @@ template <class T> struct Pyr {
     ORC_API int orc_run_sequence_##SFX(void* h, const uint16_t* frames_mm, int nframes, int W, int H,        \
                                        const float* K, const float* pose0, const int* iters, float dist,     \
                                        float angle, const float* poses_in, float* poses_out, double* ms_out,  \
-                                       long long* n_upd_out, int velocity) {                                 \
+                                       long long* n_upd_out, int velocity, double* icp_out) {                \
         auto& V = *static_cast<Volume<T>*>(h);                                                               \
         const IcpParams prm = mkIcp(iters, dist, angle);                                                     \
         const Intr KK = mkK(K);                                                                              \
@@ -209,6 +209,12 @@ template <class T> struct Pyr {
                     const Pose<T> est = velocity ? predictPose(pose, last) : pose;                           \
                     last = pose;                                                                             \
                     const IcpResult<T> r = track(L, mv.data(), mn.data(), W, H, KK, est, pose, prm);         \
+                    if (icp_out) {                                                                           \
+                        icp_out[4 * f + 0] = r.converged;                                                    \
+                        icp_out[4 * f + 1] = static_cast<double>(r.final_step);                              \
+                        icp_out[4 * f + 2] = r.inliers;                                                      \
+                        icp_out[4 * f + 3] = r.steps;                                                        \
+                    }                                                                                        \
                     pose = poseFrom<T>(r.pose);                                                              \
                     t2 = clk::now();                                                                         \
                 }                                                                                            \

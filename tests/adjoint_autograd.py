@@ -44,6 +44,18 @@ class LossProblem:
         self.res = vol.res
         self.tsdf, self.weight = vol.voxelData()
 
+    def set_obs(self, d_obs):
+        self.obs = N.device_array(N.f32(d_obs))
+
+    def depth(self, xi):
+        """Ray depths (H, W) at pose0 * exp(xi), 0 on a miss."""
+        dbuf = N.DeviceBuffer(self.W * self.H * 4)
+        T = se3_right(self.pose0, xi).astype(np.float32)
+        out = (ctypes.c_double * 8)()
+        N.check(N.lib.azu_raycast_depth_loss(self.vol.h, N.fptr(pose_cm(T)), N.fptr(self.K), self.W, self.H,
+                                             self.obs.ptr, self.step, 0, None, dbuf.ptr, out, None))
+        return dbuf.download(np.float32, (self.H, self.W))
+
     def run(self, xi, backward=False, vals=None, idx=None):
         if vals is not None:
             t = self.tsdf.reshape(-1).copy()
@@ -63,6 +75,26 @@ class LossProblem:
         if backward:
             grad = gbuf.download(np.float32, (self.res ** 3,))
         return np.array(out[:]), grad
+
+    def mask_discontinuities(self, d_obs, h):
+        """Drops the observations of pixels whose ray depth is not smooth under a +-h pose
+        change along any of the six axes (hit/miss switches at silhouettes, bracket
+        switches on a curved field): the one-sided changes d(+h) - d(0) and d(0) - d(-h)
+        of a differentiable pixel agree to first order. The loss is differentiable on the
+        rest."""
+        keep = np.asarray(d_obs) > 0
+        d0 = self.depth(np.zeros(6)).astype(np.float64)
+        keep &= d0 > 0
+        for i in range(6):
+            e = np.zeros(6)
+            e[i] = h
+            dp, dm = self.depth(e).astype(np.float64), self.depth(-e).astype(np.float64)
+            hit = (dp > 0) & (dm > 0)
+            curv = np.abs((dp - d0) - (d0 - dm))
+            keep &= hit & (curv <= 0.1 * np.abs(dp - dm) + 2e-7)
+        masked = np.where(keep, d_obs, np.float32(0)).astype(np.float32)
+        self.set_obs(masked)
+        return masked, keep
 
 
 class DepthLoss(torch.autograd.Function):

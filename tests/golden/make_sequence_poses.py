@@ -2,8 +2,8 @@
 sphere + plane + 10 boxes, orbit + jitter) as a small fixture:
 
   c2_512_poses.npz: fp64 (100, 4, 4) poses of the FP64 oracle, fp32 (100, 4, 4) poses
-  of the FP32 oracle, truth (100, 4, 4) scene poses, and the updated-voxel counts of
-  both oracles per frame.
+  of the FP32 oracle, truth (100, 4, 4) scene poses, and per frame for both oracles the
+  updated-voxel count and the ICP outcome (converged, final step, inliers, steps).
 
 The GPU tests compare the GPU loop's poses against these without running the oracle
 (minutes of CPU at 512^3). Regenerate with: python tests/golden/make_sequence_poses.py
@@ -33,6 +33,7 @@ def main(nframes=100):
         print(name, "%.1f s" % (time.time() - t0), flush=True)
         out[name] = poses.astype(np.float64)
         out[name + "_n_upd"] = nupd
+        out[name + "_icp"] = v.last_icp  # per frame: converged, final step, inliers, steps per level
         v.close()
     np.savez_compressed(os.path.join(HERE, "c2_512_poses.npz"), **out)
 
