@@ -102,7 +102,9 @@ def test_pose_gradient_finite_differences():
     h = 1e-4
     _, keep = prob.mask_discontinuities(obs, h)
     say("differentiable observed pixels", int(keep.sum()), "of", int((obs > 0).sum()))
-    assert keep.sum() > 0.8 * (obs > 0).sum()
+    # a +-1e-4 change moves the surface ~0.1 mm against the 4 mm sample lattice, so a few
+    # per cent of the rays switch bracket per axis and direction (twelve in all)
+    assert keep.sum() > 0.5 * (obs > 0).sum()
     out, _ = prob.run(np.zeros(6), backward=True)
     fd = np.zeros(6)
     for i in range(6):
