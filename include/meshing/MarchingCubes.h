@@ -1,0 +1,83 @@
+// kfusion::meshing::MarchingCubes for the MI355X backend: a header-only C++ host class
+// over the C ABI of include/azu_hip.h, shaped like the reference's mesher
+// (include/meshing/MarchingCubes.h:32-60, MeshData.h:13-40 of iphands/azu). Extraction
+// runs on the GPU and produces the CPU extractor's mesh: the same vertices, normals and
+// indices in the same order, and the same truncation rule under the triangle cap.
+// Colour is not stored by the volume, so MeshData::colors stays empty.
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <functional>
+#include <memory>
+#include <vector>
+
+#include "../azu_hip.h"
+#include "../tsdf/TSDFVolume.h"
+
+namespace kfusion {
+namespace meshing {
+
+struct MeshData {
+    std::vector<std::array<float, 3>> positions;
+    std::vector<std::array<float, 3>> normals;
+    std::vector<uint8_t> colors;
+    std::vector<uint32_t> indices;
+    bool truncated = false;
+
+    void clear() {
+        positions.clear();
+        normals.clear();
+        colors.clear();
+        indices.clear();
+        truncated = false;
+    }
+    bool empty() const { return positions.empty(); }
+    size_t triangleCount() const { return indices.size() / 3; }
+    bool hasColors() const { return !colors.empty(); }
+};
+
+using ProgressCallback = std::function<void(float)>;
+
+class MarchingCubes {
+public:
+    // The reference's CPU and GPU triangle budget (MarchingCubes.h:48-56).
+    void setMaxTriangles(size_t n) { max_triangles_ = n; }
+    size_t maxTriangles() const { return max_triangles_; }
+
+    // extractGPU: the mesh on the device, downloaded into MeshData.
+    std::shared_ptr<MeshData> extractGPU(const tsdf::TSDFVolume& volume) {
+        azu_mesh* m = nullptr;
+        detail::check(azu_mc_extract(volume.handle(), max_triangles_, &m, nullptr), "MarchingCubes::extractGPU");
+        auto out = std::make_shared<MeshData>();
+        unsigned long long nv = 0, nt = 0;
+        int tr = 0;
+        int rc = azu_mesh_info(m, &nv, &nt, &tr);
+        if (rc == AZU_OK) {
+            out->positions.resize(nv);
+            out->normals.resize(nv);
+            out->indices.resize(3 * nt);
+            out->truncated = tr != 0;
+            rc = azu_mesh_download(m, out->positions.empty() ? nullptr : out->positions[0].data(),
+                                   out->normals.empty() ? nullptr : out->normals[0].data(),
+                                   out->indices.empty() ? nullptr : out->indices.data());
+        }
+        azu_mesh_destroy(m);
+        detail::check(rc, "MarchingCubes::extractGPU download");
+        return out;
+    }
+
+    // extract (MarchingCubes.h:37-38): the same GPU extraction; the callback, if any, is
+    // called once with 1.0 from the calling thread when the mesh is complete.
+    std::shared_ptr<MeshData> extract(const tsdf::TSDFVolume& volume, ProgressCallback cb = nullptr) {
+        auto m = extractGPU(volume);
+        if (cb) cb(1.0f);
+        return m;
+    }
+
+private:
+    size_t max_triangles_ = 2000000;
+};
+
+}  // namespace meshing
+}  // namespace kfusion
