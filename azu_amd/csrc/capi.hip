@@ -799,15 +799,17 @@ int azu_tracker_result(azu_tracker* t, azu_icp_result* out, azu_stream s) {
     return AZU_OK;
 }
 
-int azu_tracker_phase_times(azu_tracker* t, double out[3], int reset) {
+int azu_tracker_phase_times(azu_tracker* t, double out[5], int reset) {
     if (!t || !out) return fail(AZU_ERR_ARG, "null argument");
     AZU_HIP_TRY(hipDeviceSynchronize(), "synchronise for phase times");
     AZU_HIP_TRY(hipMemcpy(t->host_state, t->state, icpStateBytes(), hipMemcpyDeviceToHost), "state read");
-    unsigned long long ticks[3];
+    unsigned long long ticks[5];
     icpTicksRead(t->host_state, ticks);
     out[0] = ticks[0] * 1e-5;  // s_memrealtime runs at 100 MHz
     out[1] = ticks[1] * 1e-5;
     out[2] = (double)ticks[2];
+    out[3] = (double)ticks[3];
+    out[4] = (double)ticks[4];
     if (reset) {
         AZU_HIP_TRY(launchIcpClearTicks(t->state, nullptr), "tick clear launch");
         AZU_HIP_TRY(hipDeviceSynchronize(), "tick clear");

@@ -40,8 +40,9 @@ struct IcpState {
     int stop_level;   // level whose iteration loop has ended, -1 while none
     int iterations;   // accepted steps, one byte per level (level l at bit 8*l)
     // Phase timing of the last block, in 100 MHz clock ticks summed over the launches
-    // that reached it: cross-block sum, solve; and the number of such launches.
-    unsigned long long ticks_sum, ticks_solve, ticks_n;
+    // that reached it: cross-block sum, solve; and the number of such launches. Work of
+    // the same launches: interior live pixels visited, in-bounds model projections.
+    unsigned long long ticks_sum, ticks_solve, ticks_n, work_px, work_proj;
 };
 
 struct IcpIterArgs {
@@ -423,7 +424,7 @@ __global__ void icpInit(IcpState* st, PoseArg est, PoseArg ref) {
 }
 
 __global__ void icpClearTicks(IcpState* st) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) st->ticks_sum = st->ticks_solve = st->ticks_n = 0;
+    if (threadIdx.x == 0 && blockIdx.x == 0) st->ticks_sum = st->ticks_solve = st->ticks_n = st->work_px = st->work_proj = 0;
 }
 
 // A * B for rigid 4x4 column-major poses, each entry summed left to right
@@ -636,6 +637,8 @@ __global__ __launch_bounds__(256) void icpIter(IcpIterArgs a, IcpState* __restri
         st->ticks_sum += t_sum - t_last;
         st->ticks_solve += t_end - t_sum;
         st->ticks_n += 1;
+        st->work_px += (unsigned long long)npx;
+        st->work_proj += (unsigned long long)s_sum[kIcpSlotsF + 3];
     }
 }
 
@@ -693,6 +696,8 @@ void icpTicksRead(const void* hostCopy, unsigned long long* ticks) {
     ticks[0] = s->ticks_sum;
     ticks[1] = s->ticks_solve;
     ticks[2] = s->ticks_n;
+    ticks[3] = s->work_px;
+    ticks[4] = s->work_proj;
 }
 
 // Host view of the state (copied back once per frame).

@@ -87,8 +87,46 @@ __device__ __forceinline__ void flushCounts(unsigned long long* counters, unsign
     if (newly) atomicAdd(&slot[1], newly);
 }
 
-constexpr int kChunk = 32;          // x columns per phase
+constexpr int kChunk = 32;  // x columns per tile
+constexpr int kSub = 16;    // columns whose depth gathers are issued together
 constexpr int kLdsStride = kChunk + 1;
+
+// Conservative, wave-uniform cull of the tile {xa..xb} x {y} x {za..zb} (a rectangle of
+// lattice points at fixed y): false only when no voxel of it can pass the gates, i.e.
+// the whole rectangle is behind the camera, beyond max_depth + trunc, or projects
+// wholly outside one image edge. The rectangle's camera coordinates are affine in
+// (x, z) and the perspective image of a rectangle in front of the camera is the convex
+// hull of its corners' images, so testing the four corners suffices. The margins
+// (1 mm in depth, 2 px in the image, corners at least 10 cm in front) cover the
+// rounding of the incremental x-step the exact kernel uses for each voxel (below
+// 1e-4 m after 2048 steps), so a culled voxel is one the exact path would reject.
+__device__ __forceinline__ bool tileMayUpdate(const IntFrame& f, const VolParams& p, Intr K, int W, int H,
+                                              float max_d, int xa, int xb, int y, int za, int zb) {
+    float zmin = __builtin_inff(), zmax = -__builtin_inff();
+    bool left = true, right = true, top = true, bottom = true;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const float px = p.origin[0] + (float)((i & 1) ? xb : xa) * p.vs;
+        const float py = p.origin[1] + (float)y * p.vs;
+        const float pz = p.origin[2] + (float)((i & 2) ? zb : za) * p.vs;
+        const F3 c = f3add(mulv(f.R_wc, F3{px, py, pz}), f.t_wc);
+        zmin = fminf(zmin, c.z);
+        zmax = fmaxf(zmax, c.z);
+        if (c.z > 0.1f) {
+            const float u = K.fx * c.x / c.z + K.cx + 0.5f;
+            const float v = K.fy * c.y / c.z + K.cy + 0.5f;
+            left = left && u < -2.0f;
+            right = right && u > (float)W + 2.0f;
+            top = top && v < -2.0f;
+            bottom = bottom && v > (float)H + 2.0f;
+        } else {
+            left = right = top = bottom = false;
+        }
+    }
+    if (zmax < -1e-3f) return false;
+    if (zmin > max_d + p.trunc + 1e-3f) return false;
+    return !(left || right || top || bottom);
+}
 
 template <class S>
 __global__ __launch_bounds__(256) void integrateExact(typename S::Word* __restrict__ vox, VolParams p, Intr K,
@@ -105,6 +143,7 @@ __global__ __launch_bounds__(256) void integrateExact(typename S::Word* __restri
     if (task >= ny * nzg) return;  // whole wave; the kernel uses no block barrier
     const int y = f.y0 + task % ny;
     const int zb = f.z0 + (task / ny) * 64;
+    const int ze = min(zb + 63, f.z1);
     const int z = zb + lane;
     const bool row_ok = z <= f.z1;
     const float vs = p.vs, trunc = p.trunc;
@@ -117,48 +156,79 @@ __global__ __launch_bounds__(256) void integrateExact(typename S::Word* __restri
 
     for (int xc = f.x0; xc <= f.x1; xc += kChunk) {
         const int n = min(kChunk, f.x1 - xc + 1);
-        // Phase 1: lane = row, serial over the chunk's columns.
-#pragma unroll 4
-        for (int j = 0; j < kChunk; ++j) {
-            float tn = __builtin_nanf("");
-            if (j < n) {
-                if (row_ok && c.z > 0.0f) {
-                    const float inv_z = 1.0f / c.z;
-                    const float uf = floorf(K.fx * c.x * inv_z + K.cx + 0.5f);
-                    const float vf = floorf(K.fy * c.y * inv_z + K.cy + 0.5f);
-                    if (uf >= 0.0f && uf < fW && vf >= 0.0f && vf < fH) {
-                        const float d = depth[(int)vf * W + (int)uf];
-                        if (d >= min_d && d <= max_d) {
-                            const float sdf = d - c.z;
-                            if (!(sdf < -trunc)) tn = smin(1.0f, sdf / trunc);
-                        }
+        if (!tileMayUpdate(f, p, K, W, H, max_d, xc, xc + n - 1, y, zb, ze)) {
+            // Nothing to fold; the row's camera coordinate still advances step by step.
+            for (int j = 0; j < n; ++j) c = f3add(c, f.step);
+            continue;
+        }
+        // Phase 1: lane = row, serial over the tile's columns. Per sub-chunk, the
+        // projections first, then all depth gathers at once, then the gates.
+#pragma unroll 1
+        for (int j0 = 0; j0 < kChunk; j0 += kSub) {
+            float cz[kSub], dj[kSub];
+            int pix[kSub];
+#pragma unroll
+            for (int jj = 0; jj < kSub; ++jj) {
+                pix[jj] = -1;
+                cz[jj] = c.z;
+                if (j0 + jj < n) {
+                    if (row_ok && c.z > 0.0f) {
+                        const float inv_z = 1.0f / c.z;
+                        const float uf = floorf(K.fx * c.x * inv_z + K.cx + 0.5f);
+                        const float vf = floorf(K.fy * c.y * inv_z + K.cy + 0.5f);
+                        if (uf >= 0.0f && uf < fW && vf >= 0.0f && vf < fH) pix[jj] = (int)vf * W + (int)uf;
                     }
+                    c = f3add(c, f.step);
                 }
-                c = f3add(c, f.step);
             }
-            L[lane * kLdsStride + j] = tn;
+#pragma unroll
+            for (int jj = 0; jj < kSub; ++jj) dj[jj] = depth[pix[jj] < 0 ? 0 : pix[jj]];
+#pragma unroll
+            for (int jj = 0; jj < kSub; ++jj) {
+                float tn = __builtin_nanf("");
+                const float d = dj[jj];
+                if (pix[jj] >= 0 && d >= min_d && d <= max_d) {
+                    const float sdf = d - cz[jj];
+                    if (!(sdf < -trunc)) tn = smin(1.0f, sdf / trunc);
+                }
+                L[lane * kLdsStride + j0 + jj] = tn;
+            }
         }
         waveSync();
-        // Phase 2: lane = column, two rows per instruction.
+        // Phase 2: lane = column, two rows per instruction; per half of the rows, every
+        // load is issued before the first fold.
         const int j = lane & (kChunk - 1), g = lane >> 5;
         const bool col_ok = j < n;
-#pragma unroll 8
-        for (int k = 0; k < 32; ++k) {
-            const int r = g + 2 * k;
-            const int zz = zb + r;
-            const float tn = L[r * kLdsStride + j];
-            const bool upd = col_ok && zz <= f.z1 && !__builtin_isnan(tn);
-            bool newly = false;
-            if (upd) {
-                typename S::Word* ptr = vox + ((size_t)zz * plane + (size_t)y * res + (size_t)(xc + j));
-                const typename S::Word wd = *ptr;
-                const float w_old = S::weight(wd);
-                const float t_old = S::tsdf(wd);
-                newly = (w_old == 0.0f);
-                *ptr = S::pack((t_old * w_old + tn) / (w_old + 1.0f), smin(w_old + 1.0f, p.max_w));
+        typename S::Word* col = vox + ((size_t)y * res + (size_t)(xc + j));
+#pragma unroll 1
+        for (int k0 = 0; k0 < 32; k0 += 16) {
+            float tn[16];
+            typename S::Word wd[16];
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const int r = g + 2 * (k0 + k);
+                const float t = L[r * kLdsStride + j];
+                tn[k] = (col_ok && zb + r <= f.z1) ? t : __builtin_nanf("");
             }
-            n_upd += (unsigned long long)__popcll(__ballot(upd));
-            n_new += (unsigned long long)__popcll(__ballot(newly));
+            // Lanes with nothing to fold read voxel 0 instead (a cached line), so the
+            // loads carry no branch and issue back to back.
+#pragma unroll
+            for (int k = 0; k < 16; ++k)
+                wd[k] = *(__builtin_isnan(tn[k]) ? vox : col + (size_t)(zb + g + 2 * (k0 + k)) * plane);
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const bool upd = !__builtin_isnan(tn[k]);
+                bool newly = false;
+                if (upd) {
+                    const float w_old = S::weight(wd[k]);
+                    const float t_old = S::tsdf(wd[k]);
+                    newly = (w_old == 0.0f);
+                    col[(size_t)(zb + g + 2 * (k0 + k)) * plane] =
+                        S::pack((t_old * w_old + tn[k]) / (w_old + 1.0f), smin(w_old + 1.0f, p.max_w));
+                }
+                n_upd += (unsigned long long)__popcll(__ballot(upd));
+                n_new += (unsigned long long)__popcll(__ballot(newly));
+            }
         }
         waveSync();
     }

@@ -16,6 +16,7 @@ oracle (FP32 restatement of the reference CPU path with the reference's 20 B AoS
 voxel) on the first frames of the same sequence.
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -40,7 +41,24 @@ def parse():
     ap.add_argument("--cpu-frames", type=int, default=8)
     ap.add_argument("--profile-frames", type=int, default=24)
     ap.add_argument("--only", choices=["both", "f16", "f32"], default="both")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy kernel of MIB MiB 5 times (the known-size "
+                         "reference for HBM byte counters) and exit")
     return ap.parse_args()
+
+
+def copy_calibration(mib):
+    import torch
+
+    n = mib * (1 << 20) // 4
+    src = torch.arange(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    assert bool((dst[-4:] == src[-4:]).all())
+    log("copied %d MiB x 5" % mib)
 
 
 def log(*a):
@@ -76,6 +94,8 @@ def profile_form(frames_dev, pose0, params, fmt, W, H, K, nframes):
     loop.frame(frames_dev[0])
     N.synchronize()
     loop.vol.counters(reset=True)
+    work = (ctypes.c_double * 5)()
+    N.check(N.lib.azu_tracker_phase_times(loop.tr.h, work, 1))
     N.profile_reset()
     N.profile_enable(True)
     for f in range(1, nframes):
@@ -84,8 +104,53 @@ def profile_form(frames_dev, pose0, params, fmt, W, H, K, nframes):
     prof = N.profile_read()
     N.profile_enable(False)
     upd, _ = loop.vol.counters(reset=True)
+    N.check(N.lib.azu_tracker_phase_times(loop.tr.h, work, 1))
     loop.close()
-    return prof, upd / max(nframes - 1, 1)
+    nf = max(nframes - 1, 1)
+    icp = {"iterations_run": work[2] / nf, "pixels": work[3] / nf, "projections": work[4] / nf,
+           "sum_us": 1000 * work[0] / nf, "solve_us": 1000 * work[1] / nf}
+    return prof, upd / nf, icp
+
+
+def rooflines(prof, nprof, n_upd, icp, s_v, W, H):
+    """Achieved HBM GB/s of each hot kernel from its algorithmic bytes per frame and its
+    event-timed duration per frame (equal to bytes per launch / average launch time)."""
+    out = {}
+    for k, (launches, total_ms) in prof.items():
+        ms_frame = total_ms / nprof
+        if k.startswith("integrate"):
+            # read + write of every updated voxel, plus the float depth frame read once
+            b = 2 * s_v * n_upd + 4 * W * H
+            extra = {"updated_voxels": int(n_upd)}
+        elif k == "icp_iter":
+            # live vertex + normal per interior pixel, model vertex + normal per in-bounds
+            # projection, and the per-block partials (34 doubles x 256 blocks, written and
+            # read back) of every iteration that ran
+            b = 24 * icp["pixels"] + 24 * icp["projections"] + 2 * 34 * 8 * 256 * icp["iterations_run"]
+            extra = {"iterations_run": round(icp["iterations_run"], 2), "pixels": int(icp["pixels"]),
+                     "projections": int(icp["projections"])}
+        else:
+            continue
+        gbs = b / (ms_frame * 1e-3) / 1e9
+        out[k] = dict({"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "kernel": k,
+                       "avg_launch_us": round(1000 * total_ms / launches, 2),
+                       "algorithmic_bytes_per_launch": int(b * nprof / launches)}, **extra)
+    return out
+
+
+def measured_traffic(kernel, workload, form):
+    """HBM bytes per launch of `kernel` from the rocprofv3 FETCH_SIZE / WRITE_SIZE
+    passes recorded in profiles/traffic.json (each counter scaled by what it reads for a
+    copy kernel of known size), or None when that kernel and workload were not measured."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    e = rec.get("%s/%s/%s" % (workload, form, kernel))
+    return None if e is None else int(e["hbm_bytes_per_launch"])
 
 
 def cpu_baseline(frames_mm, pose0, params, K, nframes):
@@ -113,6 +178,9 @@ def main():
               "(the multi-GPU slab decomposition is a design note, not built); use --gpus 1.", file=sys.stderr)
         sys.exit(2)
     os.environ.setdefault("OMP_PROC_BIND", "close")
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     from azu_amd import native as N
     from azu_amd import scene as sc
 
@@ -141,27 +209,20 @@ def main():
 
     # Per-kernel timing (fresh run, events on the launch stream) for the roofline.
     head = "f16" if "f16" in results else "f32"
-    prof, n_upd = profile_form(frames_dev, pose0, params, forms[head], W, H, K,
-                               min(args.profile_frames, len(frames_dev)))
+    prof, n_upd, icp = profile_form(frames_dev, pose0, params, forms[head], W, H, K,
+                                    min(args.profile_frames, len(frames_dev)))
     nprof = max(min(args.profile_frames, len(frames_dev)) - 1, 1)
     per_frame = {k: v[1] / nprof for k, v in prof.items()}
     dom = max(per_frame, key=per_frame.get)
-    launches, total_ms = prof[dom]
-    avg_ms = total_ms / launches
     s_v = 4 if head == "f16" else 8
-    roof = None
-    if dom.startswith("integrate"):
-        # algorithmic bytes per launch: read + write of every updated voxel, plus the
-        # float depth frame read once (the live depth is float metres on the device)
-        traffic_alg = 2 * s_v * n_upd + 4 * W * H
-        achieved = traffic_alg / (avg_ms * 1e-3) / 1e9
-        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
-                "avg_launch_us": round(avg_ms * 1000, 2), "algorithmic_bytes": int(traffic_alg),
-                "updated_voxels": int(n_upd)}
-    else:
-        roof = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
-                "traffic": None, "kernel": dom, "avg_launch_us": round(avg_ms * 1000, 2)}
+    roofs = rooflines(prof, nprof, n_upd, icp, s_v, W, H)
+    roof = roofs.get(dom) or {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
+                              "traffic": None, "kernel": dom,
+                              "avg_launch_us": round(1000 * prof[dom][1] / prof[dom][0], 2)}
+    traffic = measured_traffic(roof["kernel"], args.workload, head)
+    if traffic is not None:
+        roof["traffic"] = traffic
+    log("icp work per frame:", json.dumps(icp))
     kern = {k: {"launches": int(v[0]), "avg_us": round(1000 * v[1] / v[0], 2),
                 "us_per_frame": round(1000 * v[1] / nprof, 2)} for k, v in prof.items()}
     log("kernels:", json.dumps(kern))
@@ -198,6 +259,7 @@ def main():
                            if "f16" in results and "f32" in results else None),
         "max_track_err_m": r["max_track_err_m"],
         "roofline": roof,
+        "roofline_by_kernel": {k: {kk: v[kk] for kk in ("achieved", "frac", "avg_launch_us")} for k, v in roofs.items()},
         "kernels_us_per_frame": {k: v["us_per_frame"] for k, v in kern.items()},
         "cpu_baseline": cpu,
     }

@@ -217,10 +217,12 @@ int azu_tracker_predict(azu_tracker* t, const float** d_predicted, azu_stream s)
 int azu_tracker_pose_device(azu_tracker* t, const float** d_pose);
 /* Synchronises the stream and returns the result of the last track. */
 int azu_tracker_result(azu_tracker* t, azu_icp_result* out, azu_stream s);
-/* Diagnostic: device clock time (ms) of the ICP iterations' last-block phases summed
- * since the last reset: out[0] cross-block FP64 sum, out[1] solve + pose update,
- * out[2] the number of iterations that reached them. Synchronises; reset != 0 clears. */
-int azu_tracker_phase_times(azu_tracker* t, double out[3], int reset);
+/* Diagnostic, summed over the ICP iterations that ran (not skipped after a level's
+ * stop) since the last reset: out[0] device clock ms of the last block's cross-block
+ * FP64 sum, out[1] of its solve + pose update, out[2] the number of such iterations,
+ * out[3] interior live pixels they visited, out[4] their in-bounds model projections.
+ * Synchronises; reset != 0 clears. */
+int azu_tracker_phase_times(azu_tracker* t, double out[5], int reset);
 
 /* ---- per-kernel timing (HIP events on the launch stream) ---- */
 /* While enabled, every kernel launch is bracketed by events; azu_profile_read
