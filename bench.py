@@ -84,6 +84,35 @@ def run_form(frames_dev, nframes, pose0, params, fmt, W, H, K, warmup, steps):
     return dt, poses
 
 
+def run_form_pcie(frames_mm, pose0, params, fmt, W, H, K, warmup, steps):
+    """The same loop with each frame handed over in pinned host memory: one host-to-device
+    copy of the uint16 frame per step on the loop's stream, inside the timed region."""
+    from azu_amd import native as N
+    from azu_amd.pipeline import FrameLoop
+
+    n = warmup + steps
+    fb = W * H * 2
+    host = ctypes.c_void_p()
+    N.check(N.lib.azu_host_alloc(ctypes.byref(host), fb * n))
+    ctypes.memmove(host.value, np.ascontiguousarray(frames_mm[:n]).ctypes.data, fb * n)
+    slot = N.DeviceBuffer(fb)
+    loop = FrameLoop(params, fmt, W, H, K)
+    loop.reset(pose0)
+    try:
+        for f in range(n):
+            if f == warmup:
+                N.synchronize()
+                t0 = time.perf_counter()
+            N.check(N.lib.azu_memcpy_h2d(slot.ptr, host.value + f * fb, fb, None))
+            loop.frame(slot.ptr)
+        N.synchronize()
+        dt = time.perf_counter() - t0
+    finally:
+        loop.close()
+        N.lib.azu_host_free(host.value)
+    return steps / dt
+
+
 def profile_form(frames_dev, pose0, params, fmt, W, H, K, nframes):
     """Per-kernel averages over nframes of a fresh run, HIP events on the launch stream."""
     from azu_amd import native as N
@@ -207,8 +236,11 @@ def main():
         log(name, "%.1f frames/s, %.3f ms/frame, max |t - t_true| %.2f mm" % (
             results[name]["fps"], results[name]["ms_per_step"], 1000 * terr.max()))
 
-    # Per-kernel timing (fresh run, events on the launch stream) for the roofline.
     head = "f16" if "f16" in results else "f32"
+    pcie = run_form_pcie(frames_mm, pose0, params, forms[head], W, H, K, args.warmup, args.steps)
+    log("%s with per-frame upload from pinned host memory: %.1f frames/s" % (head, pcie))
+
+    # Per-kernel timing (fresh run, events on the launch stream) for the roofline.
     prof, n_upd, icp = profile_form(frames_dev, pose0, params, forms[head], W, H, K,
                                     min(args.profile_frames, len(frames_dev)))
     nprof = max(min(args.profile_frames, len(frames_dev)) - 1, 1)
@@ -258,6 +290,7 @@ def main():
         "fp16_over_fp32": (round(results["f16"]["fps"] / results["f32"]["fps"], 3)
                            if "f16" in results and "f32" in results else None),
         "max_track_err_m": r["max_track_err_m"],
+        "pcie_inclusive_fps": round(pcie, 2),
         "roofline": roof,
         "roofline_by_kernel": {k: {kk: v[kk] for kk in ("achieved", "frac", "avg_launch_us")} for k, v in roofs.items()},
         "kernels_us_per_frame": {k: v["us_per_frame"] for k, v in kern.items()},
