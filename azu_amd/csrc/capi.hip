@@ -28,6 +28,7 @@ hipError_t launchLive(bool mm, const void* src, int W, int H, Intr K, float min_
 size_t icpStateBytes();
 int icpBlocksFor(int W, int H);
 size_t icpPartialsBytes(int maxBlocks);
+size_t icpTicketBytes();
 hipError_t launchIcpInit(void* st, const PoseArg& est, const PoseArg& ref, hipStream_t s);
 hipError_t launchPredict(const float* cur, float* last, float* pred, hipStream_t s);
 hipError_t launchIcpClearTicks(void* st, hipStream_t s);
@@ -655,14 +656,14 @@ int azu_tracker_create(int W, int H, const float K[4], const azu_icp_params* p, 
     }
     ok = ok && hipMalloc(&t->state, icpStateBytes()) == hipSuccess;
     ok = ok && hipMalloc((void**)&t->partials, icpPartialsBytes(1024)) == hipSuccess;
-    ok = ok && hipMalloc((void**)&t->ticket, sizeof(unsigned)) == hipSuccess;
+    ok = ok && hipMalloc((void**)&t->ticket, icpTicketBytes()) == hipSuccess;
     ok = ok && hipHostMalloc(&t->host_state, icpStateBytes(), hipHostMallocDefault) == hipSuccess;
     ok = ok && hipMalloc((void**)&t->motion, 32 * sizeof(float)) == hipSuccess;
     if (!ok) {
         azu_tracker_destroy(t);
         return fail(AZU_ERR_ALLOC, "tracker allocation failed");
     }
-    if (hipMemset(t->ticket, 0, sizeof(unsigned)) != hipSuccess ||
+    if (hipMemset(t->ticket, 0, icpTicketBytes()) != hipSuccess ||
         hipMemset(t->state, 0, icpStateBytes()) != hipSuccess) {
         azu_tracker_destroy(t);
         return fail(AZU_ERR_HIP, "tracker initialisation failed");

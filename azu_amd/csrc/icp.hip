@@ -67,34 +67,16 @@ __device__ __forceinline__ void twoSum(float& hi, float& lo, float b_hi, float b
     lo = lo + b_lo + e;
 }
 
-// DPP lane move: lanes whose source is out of range or in a masked row read 0, which
-// is the identity of both sums below.
-template <int CTRL, int ROW_MASK = 0xf>
-__device__ __forceinline__ float dppF(float v) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, ROW_MASK,
-                                                                 0xf, true));
-}
+// DPP lane move: lanes whose source is out of range or in a masked row read 0, the
+// identity of the sum below.
 template <int CTRL, int ROW_MASK = 0xf>
 __device__ __forceinline__ int dppI(int v) {
     return __builtin_amdgcn_update_dpp(0, v, CTRL, ROW_MASK, 0xf, true);
 }
 
-// Wave64 reduction to lane 63 with the gfx9 DPP sequence: quad_perm [1,0,3,2],
+// Wave64 integer sum to lane 63 with the gfx9 DPP sequence: quad_perm [1,0,3,2],
 // quad_perm [2,3,0,1], row_shr:4, row_shr:8, row_bcast:15 (rows 1, 3), row_bcast:31
 // (rows 2, 3).
-template <int CTRL, int RM>
-__device__ __forceinline__ void dppStepPair(float& hi, float& lo) {
-    const float ph = dppF<CTRL, RM>(hi), pl = dppF<CTRL, RM>(lo);
-    twoSum(hi, lo, ph, pl);
-}
-__device__ __forceinline__ void waveSumPair(float& hi, float& lo) {
-    dppStepPair<0xb1, 0xf>(hi, lo);
-    dppStepPair<0x4e, 0xf>(hi, lo);
-    dppStepPair<0x114, 0xf>(hi, lo);
-    dppStepPair<0x118, 0xf>(hi, lo);
-    dppStepPair<0x142, 0xa>(hi, lo);
-    dppStepPair<0x143, 0xc>(hi, lo);
-}
 __device__ __forceinline__ int waveSumInt(int v) {
     v += dppI<0xb1, 0xf>(v);
     v += dppI<0x4e, 0xf>(v);
@@ -108,7 +90,9 @@ __device__ __forceinline__ int waveSumInt(int v) {
 // ---------------------------------------------------------------------------
 // FP64 small dense algebra for the single-lane solve.
 // ---------------------------------------------------------------------------
-__device__ bool cholesky6(const double* A, double* L) {
+// Lower Cholesky factor of a symmetric 6x6 and the reciprocals of its diagonal; false
+// unless every pivot is positive (the matrix is then positive definite).
+__device__ bool cholesky6(const double* A, double* L, double* rdiag) {
     for (int j = 0; j < 6; ++j) {
         double s = A[j * 6 + j];
         for (int k = 0; k < j; ++k) s -= L[j * 6 + k] * L[j * 6 + k];
@@ -116,6 +100,7 @@ __device__ bool cholesky6(const double* A, double* L) {
         const double d = sqrt(s);
         const double inv = 1.0 / d;
         L[j * 6 + j] = d;
+        rdiag[j] = inv;
         for (int i = j + 1; i < 6; ++i) {
             double t = A[i * 6 + j];
             for (int k = 0; k < j; ++k) t -= L[i * 6 + k] * L[j * 6 + k];
@@ -163,13 +148,37 @@ __device__ void jacobiMinMax(const double* Ain, double& mn, double& mx) {
     }
 }
 
-// dampingForHessian (ICPShared.h:293-307): escalate to 1.0 when min_eig < 1e-4 or
-// max_eig > 1e7 min_eig. A Cholesky certificate settles the common case without the
-// eigen-solve: with g >= max_eig (Gershgorin) and s = max(1e-4, g / 1e7), a successful
-// factorisation of A - s I proves min_eig > s, hence neither escalation condition.
-__device__ double dampingFP64(const double* A, int level) {
+// dampingForHessian (ICPShared.h:293-307): lambda is the level's base (0.01 at level
+// 0, 0.1 above) unless min_eig < 1e-4 or max_eig > 1e7 min_eig, then 1.0. A Cholesky
+// certificate settles the common case without an eigen-solve: with g >= max_eig
+// (Gershgorin) and s = max(1e-4, g / 1e7), a successful factorisation of A - s I
+// proves min_eig > s, hence neither escalation condition.
+//
+// The solve runs three candidate factorisations at once, one per lane of the last
+// block's first wave (the same code on different data, so together they cost one):
+// lane 0 the certificate A - s I, lane 1 A + base I, lane 2 A + 1.0 I; lanes 1 and 2
+// also solve for their step. Lane 0 then picks the candidate the damping rule selects.
+struct SolveLanes {
+    int ok[3];
+    int finite;
+    double x[3][6];
+};
+
+__device__ void buildSystem(const double* sums, double* A, double* b) {
+    int k = 0;
+    for (int i = 0; i < 6; ++i)
+        for (int j = i; j < 6; ++j) {
+            A[i * 6 + j] = sums[k];
+            A[j * 6 + i] = sums[k];
+            ++k;
+        }
+    for (int i = 0; i < 6; ++i) b[i] = sums[21 + i];
+}
+
+__device__ void solveLane(const double* sums, int level, int lane, SolveLanes* out) {
     const double kMin = (double)1e-4f, kCond = (double)1e7f;
-    const double base = level == 0 ? (double)0.01f : (double)0.1f;
+    double A[36], b[6];
+    buildSystem(sums, A, b);
     double g = 0.0;
     bool fin = true;
     for (int i = 0; i < 6; ++i) {
@@ -179,17 +188,27 @@ __device__ double dampingFP64(const double* A, int level) {
         g = fmax(g, r);
         for (int j = 0; j < 6; ++j) fin = fin && isfinite(A[i * 6 + j]);
     }
-    if (!fin) return 1.0;
-    const double s = fmax(kMin, g / kCond) * (1.0 + 1e-12);
-    double M[36], L[36];
-    for (int i = 0; i < 36; ++i) M[i] = A[i];
-    for (int i = 0; i < 6; ++i) M[i * 6 + i] -= s;
-    if (cholesky6(M, L)) return base;
-    double mn, mx;
-    jacobiMinMax(A, mn, mx);
-    if (!isfinite(mn) || !isfinite(mx) || mn < kMin) return 1.0;
-    if (mx > kCond * mn) return 1.0;
-    return base;
+    const double shift = lane == 0 ? -(fmax(kMin, g / kCond) * (1.0 + 1e-12))
+                                   : (lane == 1 ? (level == 0 ? (double)0.01f : (double)0.1f) : 1.0);
+    for (int i = 0; i < 6; ++i) A[i * 6 + i] += shift;
+    double L[36], r[6];
+    const bool ok = cholesky6(A, L, r);
+    if (ok && lane > 0) {
+        double y[6], x[6];
+        for (int i = 0; i < 6; ++i) {
+            double t = b[i];
+            for (int j = 0; j < i; ++j) t -= L[i * 6 + j] * y[j];
+            y[i] = t * r[i];
+        }
+        for (int i = 5; i >= 0; --i) {
+            double t = y[i];
+            for (int j = i + 1; j < 6; ++j) t -= L[j * 6 + i] * x[j];
+            x[i] = t * r[i];
+        }
+        for (int i = 0; i < 6; ++i) out->x[lane][i] = x[i];
+    }
+    out->ok[lane] = ok ? 1 : 0;
+    if (lane == 0) out->finite = fin ? 1 : 0;
 }
 
 // Nearest rotation by one-sided Jacobi SVD in FP64, U V^T with U's smallest-singular-
@@ -284,8 +303,9 @@ __device__ void projectSO3(const double* M, double* R) {
 }
 
 // One solve + pose update (trackLevel's loop body, ICPTracker.cpp:79-130), lane 0 of
-// the last block. sums: 28 FP64 slots (21 A upper, 6 b, residual) + 6 counters.
-__device__ void icpSolve(IcpState* st, const double* sums, int level, int iter) {
+// the last block once solveLane has run on lanes 0-2. sums: 28 FP64 slots (21 A upper,
+// 6 b, residual) + 6 counters.
+__device__ void icpSolve(IcpState* st, const double* sums, const SolveLanes* sl, int level, int iter) {
     if (iter == 0) {
         st->error = 0.0f;
         st->inliers = 0;
@@ -300,34 +320,24 @@ __device__ void icpSolve(IcpState* st, const double* sums, int level, int iter) 
         return;
     }
     double A[36], b[6];
-    int k = 0;
-    for (int i = 0; i < 6; ++i)
-        for (int j = i; j < 6; ++j) {
-            A[i * 6 + j] = sums[k];
-            A[j * 6 + i] = sums[k];
-            ++k;
-        }
-    for (int i = 0; i < 6; ++i) b[i] = sums[21 + i];
+    buildSystem(sums, A, b);
     const double residual = sums[27];
     for (int i = 0; i < 36; ++i) st->information[i] = (float)A[i];
-    const double lam = dampingFP64(A, level);
-    for (int i = 0; i < 6; ++i) A[i * 6 + i] += lam;
-    double L[36];
-    if (!cholesky6(A, L)) {
+    int pick = 1;  // the level's base lambda
+    if (!sl->finite) {
+        pick = 2;
+    } else if (!sl->ok[0]) {
+        const double kMin = (double)1e-4f, kCond = (double)1e7f;
+        double mn, mx;
+        jacobiMinMax(A, mn, mx);
+        if (!isfinite(mn) || !isfinite(mx) || mn < kMin || mx > kCond * mn) pick = 2;
+    }
+    if (!sl->ok[pick]) {
         st->stop_level = level;
         return;
     }
-    double y[6], xd[6];
-    for (int i = 0; i < 6; ++i) {
-        double s = b[i];
-        for (int j = 0; j < i; ++j) s -= L[i * 6 + j] * y[j];
-        y[i] = s / L[i * 6 + i];
-    }
-    for (int i = 5; i >= 0; --i) {
-        double s = y[i];
-        for (int j = i + 1; j < 6; ++j) s -= L[j * 6 + i] * xd[j];
-        xd[i] = s / L[i * 6 + i];
-    }
+    double xd[6];
+    for (int i = 0; i < 6; ++i) xd[i] = sl->x[pick][i];
     float x[6];
     bool fin = true;
     for (int i = 0; i < 6; ++i) {
@@ -471,27 +481,29 @@ struct PoseDerived {
     F3 t_cw, t_rel;
 };
 
-// The per-pixel part of buildLinearSystem (ICPTracker.cpp:236-327): accumulates one
-// live pixel's terms into (hi, lo) and its gate outcome into cnt.
-__device__ __forceinline__ void icpPixel(const IcpIterArgs& a, const PoseDerived& pd, int x, int y,
-                                         float* hi, float* lo, int* cnt) {
-    const int idx = y * a.W + x;
-    const F3 lv{a.lv[3 * idx], a.lv[3 * idx + 1], a.lv[3 * idx + 2]};
-    if (lv.z <= 0.001f || !f3finite(lv)) return;
+// The per-pixel part of buildLinearSystem (ICPTracker.cpp:236-327), in two stages so a
+// thread can issue the memory reads of several pixels together: icpProject takes a live
+// vertex to its model pixel (or -1), icpAccumulate runs the remaining gates on the
+// gathered model vertex / normal and live normal and folds the pixel's terms into
+// (hi, lo) and its gate outcome into cnt.
+__device__ __forceinline__ int icpProject(const IcpIterArgs& a, const PoseDerived& pd, F3 lv, int* cnt) {
+    if (lv.z <= 0.001f || !f3finite(lv)) return -1;
     const F3 vr = f3add(mulv(pd.R_rel, lv), pd.t_rel);
-    if (vr.z <= 0.001f) return;
+    if (vr.z <= 0.001f) return -1;
     cnt[1]++;
     const float inv_z = 1.0f / vr.z;
     const float mxf = a.Km.fx * vr.x * inv_z + a.Km.cx + 0.5f;
     const float myf = a.Km.fy * vr.y * inv_z + a.Km.cy + 0.5f;
     // floorToInt (CoordinateMath.h:27-49): finite and representable, then the bounds.
-    if (!isfinite(mxf) || !isfinite(myf)) return;
+    if (!isfinite(mxf) || !isfinite(myf)) return -1;
     const float fmx = floorf(mxf), fmy = floorf(myf);
-    if (!(fmx >= 0.0f && fmx < (float)a.MW && fmy >= 0.0f && fmy < (float)a.MH)) return;
+    if (!(fmx >= 0.0f && fmx < (float)a.MW && fmy >= 0.0f && fmy < (float)a.MH)) return -1;
     cnt[3]++;
-    const int midx = (int)fmy * a.MW + (int)fmx;
-    const F3 mv{a.mv[3 * midx], a.mv[3 * midx + 1], a.mv[3 * midx + 2]};
-    const F3 mn{a.mn[3 * midx], a.mn[3 * midx + 1], a.mn[3 * midx + 2]};
+    return (int)fmy * a.MW + (int)fmx;
+}
+
+__device__ __forceinline__ void icpAccumulate(const IcpIterArgs& a, const PoseDerived& pd, F3 lv, F3 mv, F3 mn,
+                                              F3 ln, float* hi, float* lo, int* cnt) {
     if (!(f3finite(mv) && f3dot(mv, mv) > 1e-12f) || !(f3finite(mn) && f3dot(mn, mn) > 0.9f)) return;
     cnt[2]++;
     const F3 vlw = f3add(mulv(pd.R_cw, lv), pd.t_cw);
@@ -500,7 +512,6 @@ __device__ __forceinline__ void icpPixel(const IcpIterArgs& a, const PoseDerived
         cnt[4]++;
         return;
     }
-    const F3 ln{a.ln[3 * idx], a.ln[3 * idx + 1], a.ln[3 * idx + 2]};
     if (!(f3finite(ln) && f3dot(ln, ln) > 0.9f)) return;
     const F3 nlw = mulv(pd.R_cw, ln);
     if (fabsf(f3dot(nlw, mn)) < a.angle_cos) {
@@ -533,13 +544,45 @@ __device__ __forceinline__ void icpPixel(const IcpIterArgs& a, const PoseDerived
     cnt[0]++;
 }
 
+__device__ __forceinline__ F3 load3(const float* __restrict__ p, int i) { return F3{p[3 * i], p[3 * i + 1], p[3 * i + 2]}; }
+
+// Two-level arrival ticket: block b counts in at group b % 8 (one counter per 256 B,
+// so the groups' atomics go to different channels instead of queueing on one address),
+// the last block of each group counts in at the top counter, and the last of those is
+// the grid's last block. Each arrival is a release (the block's partials are visible
+// before its count), a group's last block acquires before it releases onward, and the
+// grid's last block acquires; counters are reset by the block that completes them, for
+// the next launch. Returns true in the grid's last block (thread 0 calls it).
+constexpr int kTicketGroups = 8;
+constexpr int kTicketStride = 64;  // uints: 256 B between counters
+
+__device__ bool arriveLast(unsigned* ticket, unsigned b, unsigned nb) {
+    const unsigned ng = nb < (unsigned)kTicketGroups ? nb : (unsigned)kTicketGroups;
+    const unsigned g = b % ng;
+    const unsigned in_group = (nb - g + ng - 1) / ng;
+    unsigned* gc = ticket + (1 + g) * kTicketStride;
+    const unsigned p = __hip_atomic_fetch_add(gc, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (p != in_group - 1) return false;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    __hip_atomic_store(gc, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned q = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (q != ng - 1) return false;
+    __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return true;
+}
+
+constexpr int kPxBatch = 5;   // pixels whose reads one thread issues together
+constexpr int kFoldGroups = 9;  // block fold: threads per system slot (28 * 9 = 252 <= 256)
+
 __global__ __launch_bounds__(256) void icpIter(IcpIterArgs a, IcpState* __restrict__ st,
                                                double* __restrict__ partials, unsigned* __restrict__ ticket) {
     __shared__ PoseDerived sp;
-    __shared__ float s_hi[4][kIcpSlotsF], s_lo[4][kIcpSlotsF];
+    __shared__ float s_red[2 * kIcpSlotsF][256];  // every thread's compensated pairs, slot-major
+    __shared__ double s_part[kIcpSlotsF][kFoldGroups];
     __shared__ int s_cnt[4][kIcpSlotsI];
     __shared__ double s_grp[kSumGroups][kIcpSlots];
     __shared__ double s_sum[kIcpSlots];
+    __shared__ SolveLanes s_lanes;
     __shared__ bool s_last;
     if (st->stop_level == a.level) return;  // uniform over the grid
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -564,59 +607,87 @@ __global__ __launch_bounds__(256) void icpIter(IcpIterArgs a, IcpState* __restri
     int cnt[kIcpSlotsI] = {0, 0, 0, 0, 0, 0};
     const int iw = a.W - 2, ih = a.H - 2;
     const int npx = (iw > 0 && ih > 0) ? iw * ih : 0;
-    for (int p = blockIdx.x * 256 + tid; p < npx; p += gridDim.x * 256)
-        icpPixel(a, pd, 1 + p % iw, 1 + p / iw, hi, lo, cnt);
-
-    // Wave fold to lane 63 (compensated pairs, exact integers), then across waves.
+    const int stride = gridDim.x * 256;
+    // Each thread visits pixels p0, p0 + stride, ... in that order; per batch, all live
+    // vertex reads, then all model / live-normal reads, are issued before their use.
+    for (int p0 = blockIdx.x * 256 + tid; p0 < npx; p0 += kPxBatch * stride) {
+        int idx[kPxBatch], midx[kPxBatch];
+        F3 lv[kPxBatch];
 #pragma unroll
-    for (int i = 0; i < kIcpSlotsF; ++i) waveSumPair(hi[i], lo[i]);
+        for (int b = 0; b < kPxBatch; ++b) {
+            const int p = p0 + b * stride;
+            idx[b] = p < npx ? (1 + p / iw) * a.W + 1 + p % iw : -1;
+        }
+#pragma unroll
+        for (int b = 0; b < kPxBatch; ++b) lv[b] = load3(a.lv, idx[b] < 0 ? 0 : idx[b]);
+#pragma unroll
+        for (int b = 0; b < kPxBatch; ++b) midx[b] = idx[b] < 0 ? -1 : icpProject(a, pd, lv[b], cnt);
+        F3 mv[kPxBatch], mn[kPxBatch], ln[kPxBatch];
+#pragma unroll
+        for (int b = 0; b < kPxBatch; ++b) {
+            const int m = midx[b] < 0 ? 0 : midx[b];
+            mv[b] = load3(a.mv, m);
+            mn[b] = load3(a.mn, m);
+            ln[b] = load3(a.ln, idx[b] < 0 ? 0 : idx[b]);
+        }
+#pragma unroll
+        for (int b = 0; b < kPxBatch; ++b)
+            if (midx[b] >= 0) icpAccumulate(a, pd, lv[b], mv[b], mn[b], ln[b], hi, lo, cnt);
+    }
+
+    // Block fold: every thread's pairs go to LDS; thread (s, g) folds slot s over the
+    // threads g, g + 9, g + 18, ... (compensated), and the nine group values are added
+    // in FP64 in group order. Counters are exact integers (wave DPP sums).
+#pragma unroll
+    for (int i = 0; i < kIcpSlotsF; ++i) {
+        s_red[2 * i][tid] = hi[i];
+        s_red[2 * i + 1][tid] = lo[i];
+    }
 #pragma unroll
     for (int i = 0; i < kIcpSlotsI; ++i) cnt[i] = waveSumInt(cnt[i]);
     if (lane == 63) {
 #pragma unroll
-        for (int i = 0; i < kIcpSlotsF; ++i) {
-            s_hi[wv][i] = hi[i];
-            s_lo[wv][i] = lo[i];
-        }
-#pragma unroll
         for (int i = 0; i < kIcpSlotsI; ++i) s_cnt[wv][i] = cnt[i];
+    }
+    __syncthreads();
+    if (tid < kIcpSlotsF * kFoldGroups) {
+        const int sl = tid / kFoldGroups, g = tid % kFoldGroups;
+        float h = 0.0f, l = 0.0f;
+        for (int j = g; j < 256; j += kFoldGroups) twoSum(h, l, s_red[2 * sl][j], s_red[2 * sl + 1][j]);
+        s_part[sl][g] = (double)h + (double)l;
     }
     __syncthreads();
     const unsigned nb = gridDim.x;
     if (tid < kIcpSlots) {
         double v;
         if (tid < kIcpSlotsF) {
-            float h = s_hi[0][tid], l = s_lo[0][tid];
-            for (int w2 = 1; w2 < 4; ++w2) twoSum(h, l, s_hi[w2][tid], s_lo[w2][tid]);
-            v = (double)h + (double)l;
+            v = 0.0;
+            for (int g = 0; g < kFoldGroups; ++g) v += s_part[tid][g];
         } else {
             const int i = tid - kIcpSlotsF;
             v = (double)(s_cnt[0][i] + s_cnt[1][i] + s_cnt[2][i] + s_cnt[3][i]);
         }
-        partials[(size_t)tid * kIcpMaxBlocks + blockIdx.x] = v;
+        // Block-major (coalesced both ways). The stores come from wave 0, the wave whose
+        // thread 0 then releases them with its arrival.
+        partials[(size_t)blockIdx.x * kIcpSlots + tid] = v;
     }
-    __threadfence();
     __syncthreads();
-    if (tid == 0) {
-        const unsigned prev = atomicInc(ticket, nb - 1);  // wraps to 0 for the next launch
-        s_last = (prev == nb - 1);
-    }
+    if (tid == 0) s_last = arriveLast(ticket, blockIdx.x, nb);
     __syncthreads();
     if (!s_last) return;
     const unsigned long long t_last = __builtin_amdgcn_s_memrealtime();
-    __threadfence();
     // Last block: FP64 cross-block sum. Thread (g, s) sums slot s over blocks g, g + 7,
     // g + 14, ... with all of its loads issued before the first add; the seven group
     // sums are then added in group order. The order is fixed, so the result is too.
     if (tid < kSumGroups * kIcpSlots) {
         const int s = tid % kIcpSlots, g = tid / kIcpSlots;
-        const double* src = partials + (size_t)s * kIcpMaxBlocks;
+        const double* src = partials + s;
         constexpr int kMaxPer = (kIcpMaxBlocks + kSumGroups - 1) / kSumGroups;
         double v[kMaxPer];
 #pragma unroll
         for (int i = 0; i < kMaxPer; ++i) {
             const unsigned bi = (unsigned)(g + kSumGroups * i);
-            v[i] = bi < nb ? src[bi] : 0.0;
+            v[i] = bi < nb ? src[(size_t)bi * kIcpSlots] : 0.0;
         }
         double acc = 0.0;
 #pragma unroll
@@ -630,15 +701,19 @@ __global__ __launch_bounds__(256) void icpIter(IcpIterArgs a, IcpState* __restri
         s_sum[tid] = acc;
     }
     __syncthreads();
-    if (tid == 0) {
+    if (tid < 64) {
         const unsigned long long t_sum = __builtin_amdgcn_s_memrealtime();
-        icpSolve(st, s_sum, a.level, a.iter);
-        const unsigned long long t_end = __builtin_amdgcn_s_memrealtime();
-        st->ticks_sum += t_sum - t_last;
-        st->ticks_solve += t_end - t_sum;
-        st->ticks_n += 1;
-        st->work_px += (unsigned long long)npx;
-        st->work_proj += (unsigned long long)s_sum[kIcpSlotsF + 3];
+        if (tid < 3 && (int)s_sum[kIcpSlotsF] >= 10) solveLane(s_sum, a.level, tid, &s_lanes);
+        waveSync();
+        if (tid == 0) {
+            icpSolve(st, s_sum, &s_lanes, a.level, a.iter);
+            const unsigned long long t_end = __builtin_amdgcn_s_memrealtime();
+            st->ticks_sum += t_sum - t_last;
+            st->ticks_solve += t_end - t_sum;
+            st->ticks_n += 1;
+            st->work_px += (unsigned long long)npx;
+            st->work_proj += (unsigned long long)s_sum[kIcpSlotsF + 3];
+        }
     }
 }
 
@@ -654,6 +729,7 @@ int icpBlocksFor(int W, int H) {
     return (int)b;
 }
 size_t icpPartialsBytes(int) { return sizeof(double) * kIcpSlots * (size_t)kIcpMaxBlocks; }
+size_t icpTicketBytes() { return sizeof(unsigned) * kTicketStride * (1 + kTicketGroups); }
 
 hipError_t launchIcpInit(void* st, const PoseArg& est, const PoseArg& ref, hipStream_t s) {
     hipLaunchKernelGGL(icpInit, dim3(1), dim3(64), 0, s, (IcpState*)st, est, ref);
