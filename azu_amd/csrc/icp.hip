@@ -97,8 +97,11 @@ __device__ bool cholesky6(const double* A, double* L, double* rdiag) {
         double s = A[j * 6 + j];
         for (int k = 0; k < j; ++k) s -= L[j * 6 + k] * L[j * 6 + k];
         if (!(s > 0.0)) return false;
-        const double d = sqrt(s);
-        const double inv = 1.0 / d;
+        // 1 / sqrt(s) from the hardware estimate and one Newton step (relative error
+        // ~1e-16, far below the float step this solve produces), then sqrt(s) = s / sqrt(s).
+        const double r0 = __builtin_amdgcn_rsq(s);
+        const double inv = r0 * (1.5 - (0.5 * s) * (r0 * r0));
+        const double d = s * inv;
         L[j * 6 + j] = d;
         rdiag[j] = inv;
         for (int i = j + 1; i < 6; ++i) {
@@ -302,10 +305,18 @@ __device__ void projectSO3(const double* M, double* R) {
     for (int i = 0; i < 9; ++i) R[i] = X[i];
 }
 
+// Pose terms every thread of a launch uses, computed once per block into LDS.
+struct PoseDerived {
+    M33 R_cw, R_rel;
+    F3 t_cw, t_rel;
+    int iterations;  // the state's packed step counts when the launch began
+};
+
 // One solve + pose update (trackLevel's loop body, ICPTracker.cpp:79-130), lane 0 of
 // the last block once solveLane has run on lanes 0-2. sums: 28 FP64 slots (21 A upper,
 // 6 b, residual) + 6 counters.
-__device__ void icpSolve(IcpState* st, const double* sums, const SolveLanes* sl, int level, int iter) {
+__device__ void icpSolve(IcpState* st, const double* sums, const SolveLanes* sl, const PoseDerived& pd, int level,
+                         int iter) {
     if (iter == 0) {
         st->error = 0.0f;
         st->inliers = 0;
@@ -382,11 +393,11 @@ __device__ void icpSolve(IcpState* st, const double* sums, const SolveLanes* sl,
         Rd[2][0] = -ry;
         Rd[2][1] = rx;
     }
-    float P[3][3], t[3];
-    for (int i = 0; i < 3; ++i) {
-        for (int j = 0; j < 3; ++j) P[i][j] = st->pose[j * 4 + i];
-        t[i] = st->pose[12 + i];
-    }
+    // The pose this launch started from (read into LDS at its start; only this block
+    // writes it, after every block has read it).
+    float P[3][3], t[3] = {pd.t_cw.x, pd.t_cw.y, pd.t_cw.z};
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) P[i][j] = pd.R_cw.m[i][j];
     double Md[9];
     float nt[3];
     for (int i = 0; i < 3; ++i) {
@@ -401,7 +412,7 @@ __device__ void icpSolve(IcpState* st, const double* sums, const SolveLanes* sl,
     }
     st->final_step = step_norm;
     st->last_final_step = step_norm;
-    st->iterations += 1 << (8 * level);
+    st->iterations = pd.iterations + (1 << (8 * level));
     st->error = (float)(residual / (double)(inliers > 1 ? inliers : 1));
     st->inliers = inliers;
     if (step_norm < 5e-5f) {
@@ -434,7 +445,8 @@ __global__ void icpInit(IcpState* st, PoseArg est, PoseArg ref) {
 }
 
 __global__ void icpClearTicks(IcpState* st) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) st->ticks_sum = st->ticks_solve = st->ticks_n = st->work_px = st->work_proj = 0;
+    if (threadIdx.x == 0 && blockIdx.x == 0)
+        st->ticks_sum = st->ticks_solve = st->ticks_n = st->work_px = st->work_proj = 0;
 }
 
 // A * B for rigid 4x4 column-major poses, each entry summed left to right
@@ -476,10 +488,6 @@ __global__ void predictPose(const float* __restrict__ cur, float* __restrict__ l
     }
 }
 
-struct PoseDerived {
-    M33 R_cw, R_rel;
-    F3 t_cw, t_rel;
-};
 
 // The per-pixel part of buildLinearSystem (ICPTracker.cpp:236-327), in two stages so a
 // thread can issue the memory reads of several pixels together: icpProject takes a live
@@ -574,30 +582,42 @@ __device__ bool arriveLast(unsigned* ticket, unsigned b, unsigned nb) {
 constexpr int kPxBatch = 5;   // pixels whose reads one thread issues together
 constexpr int kFoldGroups = 9;  // block fold: threads per system slot (28 * 9 = 252 <= 256)
 
-__global__ __launch_bounds__(256) void icpIter(IcpIterArgs a, IcpState* __restrict__ st,
-                                               double* __restrict__ partials, unsigned* __restrict__ ticket) {
-    __shared__ PoseDerived sp;
-    __shared__ float s_red[2 * kIcpSlotsF][256];  // every thread's compensated pairs, slot-major
-    __shared__ double s_part[kIcpSlotsF][kFoldGroups];
-    __shared__ int s_cnt[4][kIcpSlotsI];
-    __shared__ double s_grp[kSumGroups][kIcpSlots];
-    __shared__ double s_sum[kIcpSlots];
-    __shared__ SolveLanes s_lanes;
-    __shared__ bool s_last;
-    if (st->stop_level == a.level) return;  // uniform over the grid
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    if (tid == 0) {
-        const Rigid pose = rigidFromPtr(st->pose);
-        const Rigid ref = rigidFromPtr(st->ref);
-        const Rigid rinv = rigidInverse(ref);
-        sp.R_cw = pose.R;
-        sp.t_cw = pose.t;
-        sp.R_rel = mulm(rinv.R, pose.R);
-        sp.t_rel = f3add(mulv(rinv.R, pose.t), rinv.t);
-    }
-    __syncthreads();
-    const PoseDerived pd = sp;
+// Blocks an iteration over a W x H level is split into: one per 256 interior pixels,
+// at most kIcpMaxBlocks.
+__host__ __device__ int icpBlocksFor(int W, int H) {
+    const long long npx = (long long)(W > 2 ? W - 2 : 0) * (H > 2 ? H - 2 : 0);
+    long long b = (npx + 255) / 256;
+    if (b < 1) b = 1;
+    if (b > kIcpMaxBlocks) b = kIcpMaxBlocks;
+    return (int)b;
+}
 
+struct IcpShared {
+    PoseDerived sp;
+    float red[2 * kIcpSlotsF][256];  // every thread's compensated pairs, slot-major
+    double part[kIcpSlotsF][kFoldGroups];
+    int cnt[4][kIcpSlotsI];
+    double grp[kSumGroups][kIcpSlots];
+    double sum[kIcpSlots];
+    SolveLanes lanes;
+    int last;
+};
+
+__device__ __forceinline__ void derivePose(IcpShared& sh, const Rigid& pose, const Rigid& rinv, int iterations) {
+    sh.sp.R_cw = pose.R;
+    sh.sp.t_cw = pose.t;
+    sh.sp.R_rel = mulm(rinv.R, pose.R);
+    sh.sp.t_rel = f3add(mulv(rinv.R, pose.t), rinv.t);
+    sh.sp.iterations = iterations;
+}
+
+// One block's share of an iteration: the pixel pass over its pixels, the block fold and
+// its FP64 partials (written by wave 0). The level is split over nblk blocks (blocks
+// from nblk on have no pixels and write nothing). Returns the level's interior pixel
+// count.
+__device__ int icpBlockPass(const IcpIterArgs& a, const PoseDerived& pd, double* __restrict__ partials,
+                            IcpShared& sh, int nblk) {
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     float hi[kIcpSlotsF], lo[kIcpSlotsF];
 #pragma unroll
     for (int i = 0; i < kIcpSlotsF; ++i) {
@@ -607,7 +627,8 @@ __global__ __launch_bounds__(256) void icpIter(IcpIterArgs a, IcpState* __restri
     int cnt[kIcpSlotsI] = {0, 0, 0, 0, 0, 0};
     const int iw = a.W - 2, ih = a.H - 2;
     const int npx = (iw > 0 && ih > 0) ? iw * ih : 0;
-    const int stride = gridDim.x * 256;
+    if ((int)blockIdx.x >= nblk) return npx;
+    const int stride = nblk * 256;
     // Each thread visits pixels p0, p0 + stride, ... in that order; per batch, all live
     // vertex reads, then all model / live-normal reads, are issued before their use.
     for (int p0 = blockIdx.x * 256 + tid; p0 < npx; p0 += kPxBatch * stride) {
@@ -640,45 +661,48 @@ __global__ __launch_bounds__(256) void icpIter(IcpIterArgs a, IcpState* __restri
     // in FP64 in group order. Counters are exact integers (wave DPP sums).
 #pragma unroll
     for (int i = 0; i < kIcpSlotsF; ++i) {
-        s_red[2 * i][tid] = hi[i];
-        s_red[2 * i + 1][tid] = lo[i];
+        sh.red[2 * i][tid] = hi[i];
+        sh.red[2 * i + 1][tid] = lo[i];
     }
 #pragma unroll
     for (int i = 0; i < kIcpSlotsI; ++i) cnt[i] = waveSumInt(cnt[i]);
     if (lane == 63) {
 #pragma unroll
-        for (int i = 0; i < kIcpSlotsI; ++i) s_cnt[wv][i] = cnt[i];
+        for (int i = 0; i < kIcpSlotsI; ++i) sh.cnt[wv][i] = cnt[i];
     }
     __syncthreads();
     if (tid < kIcpSlotsF * kFoldGroups) {
         const int sl = tid / kFoldGroups, g = tid % kFoldGroups;
         float h = 0.0f, l = 0.0f;
-        for (int j = g; j < 256; j += kFoldGroups) twoSum(h, l, s_red[2 * sl][j], s_red[2 * sl + 1][j]);
-        s_part[sl][g] = (double)h + (double)l;
+        for (int j = g; j < 256; j += kFoldGroups) twoSum(h, l, sh.red[2 * sl][j], sh.red[2 * sl + 1][j]);
+        sh.part[sl][g] = (double)h + (double)l;
     }
     __syncthreads();
-    const unsigned nb = gridDim.x;
     if (tid < kIcpSlots) {
         double v;
         if (tid < kIcpSlotsF) {
             v = 0.0;
-            for (int g = 0; g < kFoldGroups; ++g) v += s_part[tid][g];
+            for (int g = 0; g < kFoldGroups; ++g) v += sh.part[tid][g];
         } else {
             const int i = tid - kIcpSlotsF;
-            v = (double)(s_cnt[0][i] + s_cnt[1][i] + s_cnt[2][i] + s_cnt[3][i]);
+            v = (double)(sh.cnt[0][i] + sh.cnt[1][i] + sh.cnt[2][i] + sh.cnt[3][i]);
         }
         // Block-major (coalesced both ways). The stores come from wave 0, the wave whose
         // thread 0 then releases them with its arrival.
         partials[(size_t)blockIdx.x * kIcpSlots + tid] = v;
     }
-    __syncthreads();
-    if (tid == 0) s_last = arriveLast(ticket, blockIdx.x, nb);
-    __syncthreads();
-    if (!s_last) return;
-    const unsigned long long t_last = __builtin_amdgcn_s_memrealtime();
-    // Last block: FP64 cross-block sum. Thread (g, s) sums slot s over blocks g, g + 7,
-    // g + 14, ... with all of its loads issued before the first add; the seven group
-    // sums are then added in group order. The order is fixed, so the result is too.
+    return npx;
+}
+
+// The grid's last block: FP64 cross-block sum, then the solve and state update. Thread
+// (g, s) sums slot s over blocks g, g + 7, g + 14, ... with all of its loads issued
+// before the first add; the seven group sums are then added in group order. The order
+// is fixed, so the result is too.
+__device__ void icpLastBlock(IcpState* __restrict__ st, const double* __restrict__ partials, IcpShared& sh,
+                             const PoseDerived& pd, int level, int iter, int npx, int nblk,
+                             unsigned long long t_last) {
+    const int tid = threadIdx.x;
+    const unsigned nb = (unsigned)nblk;
     if (tid < kSumGroups * kIcpSlots) {
         const int s = tid % kIcpSlots, g = tid / kIcpSlots;
         const double* src = partials + s;
@@ -692,42 +716,56 @@ __global__ __launch_bounds__(256) void icpIter(IcpIterArgs a, IcpState* __restri
         double acc = 0.0;
 #pragma unroll
         for (int i = 0; i < kMaxPer; ++i) acc += v[i];
-        s_grp[g][s] = acc;
+        sh.grp[g][s] = acc;
     }
     __syncthreads();
     if (tid < kIcpSlots) {
         double acc = 0.0;
-        for (int g = 0; g < kSumGroups; ++g) acc += s_grp[g][tid];
-        s_sum[tid] = acc;
+        for (int g = 0; g < kSumGroups; ++g) acc += sh.grp[g][tid];
+        sh.sum[tid] = acc;
     }
     __syncthreads();
     if (tid < 64) {
         const unsigned long long t_sum = __builtin_amdgcn_s_memrealtime();
-        if (tid < 3 && (int)s_sum[kIcpSlotsF] >= 10) solveLane(s_sum, a.level, tid, &s_lanes);
+        if (tid < 3 && (int)sh.sum[kIcpSlotsF] >= 10) solveLane(sh.sum, level, tid, &sh.lanes);
         waveSync();
         if (tid == 0) {
-            icpSolve(st, s_sum, &s_lanes, a.level, a.iter);
+            icpSolve(st, sh.sum, &sh.lanes, pd, level, iter);
             const unsigned long long t_end = __builtin_amdgcn_s_memrealtime();
             st->ticks_sum += t_sum - t_last;
             st->ticks_solve += t_end - t_sum;
             st->ticks_n += 1;
             st->work_px += (unsigned long long)npx;
-            st->work_proj += (unsigned long long)s_sum[kIcpSlotsF + 3];
+            st->work_proj += (unsigned long long)sh.sum[kIcpSlotsF + 3];
         }
     }
+}
+
+// One ICP iteration per launch.
+__global__ __launch_bounds__(256) void icpIter(IcpIterArgs a, IcpState* __restrict__ st,
+                                               double* __restrict__ partials, unsigned* __restrict__ ticket) {
+    __shared__ IcpShared sh;
+    if (st->stop_level == a.level) return;  // uniform over the grid
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        const Rigid ref = rigidFromPtr(st->ref);
+        derivePose(sh, rigidFromPtr(st->pose), rigidInverse(ref), st->iterations);
+    }
+    __syncthreads();
+    const PoseDerived pd = sh.sp;
+    const int npx = icpBlockPass(a, pd, partials, sh, gridDim.x);
+    __syncthreads();
+    if (tid == 0) sh.last = arriveLast(ticket, blockIdx.x, gridDim.x) ? 1 : 0;
+    __syncthreads();
+    if (!sh.last) return;
+    icpLastBlock(st, partials, sh, pd, a.level, a.iter, npx, gridDim.x, __builtin_amdgcn_s_memrealtime());
 }
 
 // ---------------------------------------------------------------------------
 // Host launchers.
 // ---------------------------------------------------------------------------
 size_t icpStateBytes() { return sizeof(IcpState); }
-int icpBlocksFor(int W, int H) {
-    const long long npx = (long long)(W > 2 ? W - 2 : 0) * (H > 2 ? H - 2 : 0);
-    long long b = (npx + 255) / 256;
-    if (b < 1) b = 1;
-    if (b > kIcpMaxBlocks) b = kIcpMaxBlocks;
-    return (int)b;
-}
+
 size_t icpPartialsBytes(int) { return sizeof(double) * kIcpSlots * (size_t)kIcpMaxBlocks; }
 size_t icpTicketBytes() { return sizeof(unsigned) * kTicketStride * (1 + kTicketGroups); }
 

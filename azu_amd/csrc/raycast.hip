@@ -15,6 +15,13 @@
 // multiplications by 1/vs and fmaf interpolation.
 #include "common.hpp"
 
+#ifndef RAY_SPEC32
+#define RAY_SPEC32 4
+#endif
+#ifndef RAY_SPEC16
+#define RAY_SPEC16 0
+#endif
+
 namespace azu {
 
 template <class S> struct PairIO;
@@ -46,9 +53,19 @@ template <class S> struct Vol {
     float ox, oy, oz;
 };
 
-// sampleTSDF (TSDFVolume.cpp:492-525)
+// sampleTSDF (TSDFVolume.cpp:492-525) in two halves, so a ray can issue the reads of
+// several samples before it evaluates the first: sampleLoad locates the cell and reads
+// its 8 corners (nothing is read outside the volume), sampleEval applies the weight
+// gate and interpolates.
+template <class S> struct Cell {
+    typename PairIO<S>::P q0, q1, q2, q3;
+    float tx, ty, tz;
+    bool inside;
+};
+
 template <class S, bool E>
-__device__ __forceinline__ bool sampleVol(const Vol<S>& V, F3 p, float& f) {
+__device__ __forceinline__ Cell<S> sampleLoad(const Vol<S>& V, F3 p) {
+    Cell<S> c;
     float gx, gy, gz;
     if (E) {
         gx = (p.x - V.ox) / V.vs;
@@ -61,21 +78,31 @@ __device__ __forceinline__ bool sampleVol(const Vol<S>& V, F3 p, float& f) {
     }
     const float fx0 = floorf(gx), fy0 = floorf(gy), fz0 = floorf(gz);
     const float top = (float)(V.res - 1);
-    if (!(fx0 >= 0.0f && fy0 >= 0.0f && fz0 >= 0.0f && fx0 < top && fy0 < top && fz0 < top)) return false;
-    const int x0 = (int)fx0, y0 = (int)fy0, z0 = (int)fz0;
-    const float tx = gx - fx0, ty = gy - fy0, tz = gz - fz0;
+    c.inside = fx0 >= 0.0f && fy0 >= 0.0f && fz0 >= 0.0f && fx0 < top && fy0 < top && fz0 < top;
+    const int x0 = c.inside ? (int)fx0 : 0, y0 = c.inside ? (int)fy0 : 0, z0 = c.inside ? (int)fz0 : 0;
+    c.tx = gx - fx0;
+    c.ty = gy - fy0;
+    c.tz = gz - fz0;
     const size_t sy = (size_t)V.res, sz = (size_t)V.res * (size_t)V.res;
     const size_t b = (size_t)z0 * sz + (size_t)y0 * sy + (size_t)x0;
     using IO = PairIO<S>;
-    const typename IO::P q0 = IO::load(V.vox + b);
-    const typename IO::P q1 = IO::load(V.vox + b + sy);
-    const typename IO::P q2 = IO::load(V.vox + b + sz);
-    const typename IO::P q3 = IO::load(V.vox + b + sy + sz);
-    if (!(S::weight(q0.a) > 0.0f && S::weight(q0.b) > 0.0f && S::weight(q1.a) > 0.0f && S::weight(q1.b) > 0.0f &&
-          S::weight(q2.a) > 0.0f && S::weight(q2.b) > 0.0f && S::weight(q3.a) > 0.0f && S::weight(q3.b) > 0.0f))
+    c.q0 = IO::load(V.vox + b);
+    c.q1 = IO::load(V.vox + b + sy);
+    c.q2 = IO::load(V.vox + b + sz);
+    c.q3 = IO::load(V.vox + b + sy + sz);
+    return c;
+}
+
+template <class S, bool E>
+__device__ __forceinline__ bool sampleEval(const Cell<S>& c, float& f) {
+    if (!c.inside) return false;
+    if (!(S::weight(c.q0.a) > 0.0f && S::weight(c.q0.b) > 0.0f && S::weight(c.q1.a) > 0.0f &&
+          S::weight(c.q1.b) > 0.0f && S::weight(c.q2.a) > 0.0f && S::weight(c.q2.b) > 0.0f &&
+          S::weight(c.q3.a) > 0.0f && S::weight(c.q3.b) > 0.0f))
         return false;
-    const float c0 = S::tsdf(q0.a), c1 = S::tsdf(q0.b), c2 = S::tsdf(q1.a), c3 = S::tsdf(q1.b);
-    const float c4 = S::tsdf(q2.a), c5 = S::tsdf(q2.b), c6 = S::tsdf(q3.a), c7 = S::tsdf(q3.b);
+    const float tx = c.tx, ty = c.ty, tz = c.tz;
+    const float c0 = S::tsdf(c.q0.a), c1 = S::tsdf(c.q0.b), c2 = S::tsdf(c.q1.a), c3 = S::tsdf(c.q1.b);
+    const float c4 = S::tsdf(c.q2.a), c5 = S::tsdf(c.q2.b), c6 = S::tsdf(c.q3.a), c7 = S::tsdf(c.q3.b);
     if (E) {
         const float v00 = c0 * (1.0f - tx) + c1 * tx;
         const float v10 = c2 * (1.0f - tx) + c3 * tx;
@@ -91,6 +118,36 @@ __device__ __forceinline__ bool sampleVol(const Vol<S>& V, F3 p, float& f) {
         f = fmaf(v1 - v0, tz, v0);
     }
     return true;
+}
+
+template <class S, bool E>
+__device__ __forceinline__ bool sampleVol(const Vol<S>& V, F3 p, float& f) {
+    float gx, gy, gz;
+    if (E) {
+        gx = (p.x - V.ox) / V.vs;
+        gy = (p.y - V.oy) / V.vs;
+        gz = (p.z - V.oz) / V.vs;
+    } else {
+        gx = (p.x - V.ox) * V.inv_vs;
+        gy = (p.y - V.oy) * V.inv_vs;
+        gz = (p.z - V.oz) * V.inv_vs;
+    }
+    const float fx0 = floorf(gx), fy0 = floorf(gy), fz0 = floorf(gz);
+    const float top = (float)(V.res - 1);
+    if (!(fx0 >= 0.0f && fy0 >= 0.0f && fz0 >= 0.0f && fx0 < top && fy0 < top && fz0 < top)) return false;
+    Cell<S> c;
+    c.inside = true;
+    c.tx = gx - fx0;
+    c.ty = gy - fy0;
+    c.tz = gz - fz0;
+    const size_t sy = (size_t)V.res, sz = (size_t)V.res * (size_t)V.res;
+    const size_t b = (size_t)(int)fz0 * sz + (size_t)(int)fy0 * sy + (size_t)(int)fx0;
+    using IO = PairIO<S>;
+    c.q0 = IO::load(V.vox + b);
+    c.q1 = IO::load(V.vox + b + sy);
+    c.q2 = IO::load(V.vox + b + sz);
+    c.q3 = IO::load(V.vox + b + sy + sz);
+    return sampleEval<S, E>(c, f);
 }
 
 // computeNormal (TSDFVolume.cpp:478-490)
@@ -117,6 +174,9 @@ struct RayOut {
     float* norms;
     float* t_hit;  // optional: ray parameter of the hit (NaN on a miss)
 };
+
+// Far-step samples whose reads one ray issues together, per storage form.
+template <class S> constexpr int kSpecOf = S::kBytes == 8 ? RAY_SPEC32 : RAY_SPEC16;
 
 template <class S, bool E>
 __global__ __launch_bounds__(256) void raycastKernel(Vol<S> V, VolParams p, PoseArg pa, Intr K, int W, int H,
@@ -174,7 +234,60 @@ __global__ __launch_bounds__(256) void raycastKernel(Vol<S> V, VolParams p, Pose
         bool prev_valid = sampleVol<S, E>(V, f3add(cam, f3mul(dir, t_prev)), f_prev);
         bool go = !(prev_valid && (!isfinite(f_prev) || f_prev < 0.0f));
         while (go && t_prev < t_far) {
-            const float step = (!prev_valid || f_prev >= 0.999f) ? far_step : smax(min_step, 0.8f * f_prev * trunc);
+            if (kSpecOf<S> > 0 && (!prev_valid || f_prev >= 0.999f)) {
+                // A run of far steps: while every sample is unobserved or >= 0.999 the
+                // step stays far_step, so the next kSpec parameters follow from t_prev
+                // alone (the same recurrence as one step at a time). Their reads are
+                // issued together, then the samples are taken in order; the run ends at
+                // the first sample that changes the step, and the rest are discarded.
+                constexpr int kSpec = kSpecOf<S> > 0 ? kSpecOf<S> : 1;
+                float ts[kSpec];
+                Cell<S> cs[kSpec];
+                float tt = t_prev;
+#pragma unroll
+                for (int k = 0; k < kSpec; ++k) {
+                    tt = smin(tt + far_step, t_far);
+                    ts[k] = tt;
+                }
+#pragma unroll
+                for (int k = 0; k < kSpec; ++k) cs[k] = sampleLoad<S, E>(V, f3add(cam, f3mul(dir, ts[k])));
+#pragma unroll
+                for (int k = 0; k < kSpec; ++k) {
+                    const float t = ts[k];
+                    float f = 1.0f;
+                    const bool valid = sampleEval<S, E>(cs[k], f);
+                    if (valid && !isfinite(f)) {
+                        go = false;
+                        break;
+                    }
+                    if (valid && prev_valid) {
+                        if (f_prev > 0.0f && f <= 0.0f) {
+                            t_hit = t_prev + (t - t_prev) * (f_prev / (f_prev - f));
+                            found = true;
+                            go = false;
+                            break;
+                        }
+                        if (f_prev < 0.0f && f >= 0.0f) {
+                            go = false;
+                            break;
+                        }
+                    } else if (valid && f < 0.0f) {
+                        go = false;
+                        break;
+                    }
+                    t_prev = t;
+                    f_prev = f;
+                    prev_valid = valid;
+                    if (t >= t_far) {
+                        go = false;
+                        break;
+                    }
+                    if (valid && f < 0.999f) break;
+                }
+                continue;
+            }
+            const float step =
+                (!prev_valid || f_prev >= 0.999f) ? far_step : smax(min_step, 0.8f * f_prev * trunc);
             const float t = smin(t_prev + step, t_far);
             float f = 1.0f;
             const bool valid = sampleVol<S, E>(V, f3add(cam, f3mul(dir, t)), f);
