@@ -18,11 +18,12 @@
 
 namespace azu {
 hipError_t launchIntegrate(int format, bool exact, void* vox, const VolParams& p, Intr K, int W, int H,
-                           const float* depth, float min_d, float max_d, const PoseArg& pa,
-                           unsigned long long* counters, hipStream_t st);
+                           const float* depth, float min_d, float max_d, const PoseArg& pa, const FixedDiv& dtr,
+                           bool recip, unsigned long long* counters, hipStream_t st);
 hipError_t launchFillEmpty(int format, void* vox, size_t n, hipStream_t st);
-hipError_t launchRaycast(int format, bool exact, const void* vox, const VolParams& p, const PoseArg& pa, Intr K,
-                         int W, int H, float* verts, float* norms, float* t_hit, hipStream_t st);
+hipError_t launchRaycast(int format, bool exact, const void* vox, const VolParams& p, const FixedDiv& dvs,
+                         const PoseArg& pa, Intr K, int W, int H, float* verts, float* norms, float* t_hit,
+                         hipStream_t st);
 hipError_t launchLive(bool mm, const void* src, int W, int H, Intr K, float min_d, float max_d, float* const d[3],
                       float* const v[3], float* const n[3], hipStream_t st);
 size_t icpStateBytes();
@@ -201,6 +202,8 @@ struct azu_volume {
     size_t adj_bytes = 0;
     double* adj_out = nullptr;               // depth-loss result [8]
     int frames = 0;
+    FixedDiv dvs{1.0f, 1.0f, 0}, dtr{1.0f, 1.0f, 0};  // division by voxel size / truncation
+    bool recip = false;                                // verified Newton-Raphson 1 / x
     bool exact() const { return mode != AZU_MODE_FAST; }
 };
 
@@ -336,6 +339,9 @@ int azu_vol_create(const azu_tsdf_params* p, int format, azu_volume** out) {
         const hipError_t e = hipStreamSynchronize(nullptr);
         if (e != hipSuccess) rc = hipFail(e, "volume reset");
     }
+    v->dvs = makeFixedDiv(v->params.voxel_size);
+    v->dtr = makeFixedDiv(v->params.truncation);
+    v->recip = recipVerified();
     if (rc != AZU_OK) {
         azu_vol_destroy(v);
         return rc;
@@ -383,6 +389,16 @@ int azu_vol_set_params(azu_volume* v, const azu_tsdf_params* p) {
     const int rc = azu_vol_reset(v, nullptr);
     if (rc != AZU_OK) return rc;
     AZU_HIP_TRY(hipStreamSynchronize(nullptr), "volume reset");
+    v->dvs = makeFixedDiv(v->params.voxel_size);
+    v->dtr = makeFixedDiv(v->params.truncation);
+    return AZU_OK;
+}
+
+int azu_vol_division_check(const azu_volume* v, int* voxel_size_fast, int* truncation_fast, int* reciprocal_fast) {
+    if (!v) return fail(AZU_ERR_ARG, "null volume");
+    if (voxel_size_fast) *voxel_size_fast = v->dvs.fast;
+    if (truncation_fast) *truncation_fast = v->dtr.fast;
+    if (reciprocal_fast) *reciprocal_fast = v->recip ? 1 : 0;
     return AZU_OK;
 }
 
@@ -481,7 +497,7 @@ static int integrateImpl(azu_volume* v, const float* d_depth, int W, int H, cons
                        : (v->format == AZU_FMT_F32 ? "integrate_fast_f32" : "integrate_fast_f16"),
                  (hipStream_t)s);
     AZU_HIP_TRY(launchIntegrate(v->format, exact, v->vox, toVol(v->params), Intr{K[0], K[1], K[2], K[3]}, W, H,
-                                d_depth, min_d, max_d, pa, v->counters, (hipStream_t)s),
+                                d_depth, min_d, max_d, pa, v->dtr, v->recip, v->counters, (hipStream_t)s),
                 "integrate launch");
     v->frames += 1;
     return AZU_OK;
@@ -508,8 +524,8 @@ static int raycastImpl(azu_volume* v, const PoseArg& pa, const float* K, int W, 
     ProfScope ps(exact ? (v->format == AZU_FMT_F32 ? "raycast_exact_f32" : "raycast_exact_f16")
                        : (v->format == AZU_FMT_F32 ? "raycast_fast_f32" : "raycast_fast_f16"),
                  (hipStream_t)s);
-    AZU_HIP_TRY(launchRaycast(v->format, exact, v->vox, toVol(v->params), pa, Intr{K[0], K[1], K[2], K[3]}, W, H,
-                              dv, dn, dt, (hipStream_t)s),
+    AZU_HIP_TRY(launchRaycast(v->format, exact, v->vox, toVol(v->params), v->dvs, pa, Intr{K[0], K[1], K[2], K[3]},
+                              W, H, dv, dn, dt, (hipStream_t)s),
                 "raycast launch");
     return AZU_OK;
 }

@@ -50,6 +50,12 @@ struct StoreF32 {
     static constexpr int kBytes = 8;
     __device__ __forceinline__ static float tsdf(Word w) { return w.x; }
     __device__ __forceinline__ static float weight(Word w) { return w.y; }
+    // every weight > 0 (a NaN weight fails, as in the reference's !(w > 0) test)
+    __device__ __forceinline__ static bool observed8(Word a, Word b, Word c, Word d, Word e, Word f, Word g,
+                                                     Word h) {
+        return (a.y > 0.0f) & (b.y > 0.0f) & (c.y > 0.0f) & (d.y > 0.0f) & (e.y > 0.0f) & (f.y > 0.0f) &
+               (g.y > 0.0f) & (h.y > 0.0f);
+    }
     __device__ __forceinline__ static Word pack(float t, float wt) { return make_float2(t, wt); }
     __host__ __device__ static Word empty() { return float2{1.0f, 0.0f}; }
 };
@@ -61,12 +67,71 @@ struct StoreF16 {
         return __half2float(__ushort_as_half(static_cast<unsigned short>(w & 0xffffu)));
     }
     __device__ __forceinline__ static float weight(Word w) { return static_cast<float>(w >> 16); }
+    // every weight > 0: the weight is the high half, so the smallest word is >= 2^16
+    __device__ __forceinline__ static bool observed8(Word a, Word b, Word c, Word d, Word e, Word f, Word g,
+                                                     Word h) {
+        return min(min(min(a, b), min(c, d)), min(min(e, f), min(g, h))) >= 0x10000u;
+    }
     __device__ __forceinline__ static Word pack(float t, float wt) {
         const uint32_t h = __half_as_ushort(__float2half_rn(t));
         return h | (static_cast<uint32_t>(wt) << 16);
     }
     __host__ __device__ static Word empty() { return 0x3C00u; }  // half(1.0), weight 0
 };
+
+// ---------------------------------------------------------------------------
+// Division by a divisor fixed for a whole launch (the voxel size, the truncation).
+// divFixed computes q = x r (r = 1/b rounded to float) and corrects it twice with the
+// exact residual x - q b (Markstein: once q is within an ulp, a correction with the
+// correctly rounded reciprocal gives the correctly rounded quotient; the argument
+// fails where the quotient underflows). It replaces the IEEE quotient x / b only after
+// verifyFixedDivKernel has compared the two on the device for every float with
+// 2^-100 <= |x| < 2^32 and found them equal. Smaller |x| takes the IEEE division; a
+// larger |x| only arises for a sample or distance that both forms reject or clamp (far
+// outside the volume, or sdf / trunc > 1 folded as 1).
+// ---------------------------------------------------------------------------
+struct FixedDiv {
+    float b, r;
+    int fast;  // verified on this device
+};
+
+__device__ __forceinline__ float divFixed(const FixedDiv& d, float x) {
+    const float q0 = x * d.r;
+    const float q1 = fmaf(fmaf(-q0, d.b, x), d.r, q0);
+    return fmaf(fmaf(-q1, d.b, x), d.r, q1);
+}
+
+template <bool FD>
+__device__ __forceinline__ float divBy(const FixedDiv& d, float x) {
+    if (!FD) return x / d.b;
+    return fabsf(x) >= 0x1p-100f ? divFixed(d, x) : x / d.b;
+}
+
+// Host: the FixedDiv for divisor b, verified on the current device (fast = 0 when the
+// check fails or cannot run).
+FixedDiv makeFixedDiv(float b);
+
+// 1 / x from the hardware reciprocal estimate and two Newton-Raphson steps with fused
+// multiply-adds. For 2^-126 <= |x| < 2^64 it replaces the IEEE 1.0f / x once
+// verifyRecipKernel has found the two equal over that whole range on this device
+// (recipVerified); outside it (subnormal x, or results that underflow) the IEEE
+// division is used.
+__device__ __forceinline__ float recipNR(float x) {
+    float r = __builtin_amdgcn_rcpf(x);
+    r = fmaf(fmaf(-x, r, 1.0f), r, r);
+    return fmaf(fmaf(-x, r, 1.0f), r, r);
+}
+
+template <bool RC>
+__device__ __forceinline__ float recipBy(float x) {
+    if (!RC) return 1.0f / x;
+    const float a = fabsf(x);
+    return (a >= 0x1p-126f && a < 0x1p64f) ? recipNR(x) : 1.0f / x;
+}
+
+// Host: whether recipNR equals 1.0f / x for every float 2^-126 <= |x| < 2^64 on the
+// current device (checked once per device and remembered).
+bool recipVerified();
 
 // ---------------------------------------------------------------------------
 // Small math in the fixed evaluation order.

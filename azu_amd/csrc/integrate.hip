@@ -22,6 +22,9 @@
 // run with no host synchronisation. Updated / newly observed voxel counts are
 // accumulated per wave (ballot popcount) into a 2-entry device counter when given.
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <cmath>
 
 #include "common.hpp"
 
@@ -128,10 +131,10 @@ __device__ __forceinline__ bool tileMayUpdate(const IntFrame& f, const VolParams
     return !(left || right || top || bottom);
 }
 
-template <class S>
+template <class S, bool FD, bool RC>
 __global__ __launch_bounds__(256) void integrateExact(typename S::Word* __restrict__ vox, VolParams p, Intr K,
                                                       int W, int H, const float* __restrict__ depth, float min_d,
-                                                      float max_d, PoseArg pa,
+                                                      float max_d, PoseArg pa, FixedDiv dtr,
                                                       unsigned long long* __restrict__ counters) {
     __shared__ float lds[4][64 * kLdsStride];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -173,7 +176,7 @@ __global__ __launch_bounds__(256) void integrateExact(typename S::Word* __restri
                 cz[jj] = c.z;
                 if (j0 + jj < n) {
                     if (row_ok && c.z > 0.0f) {
-                        const float inv_z = 1.0f / c.z;
+                        const float inv_z = recipBy<RC>(c.z);
                         const float uf = floorf(K.fx * c.x * inv_z + K.cx + 0.5f);
                         const float vf = floorf(K.fy * c.y * inv_z + K.cy + 0.5f);
                         if (uf >= 0.0f && uf < fW && vf >= 0.0f && vf < fH) pix[jj] = (int)vf * W + (int)uf;
@@ -189,7 +192,7 @@ __global__ __launch_bounds__(256) void integrateExact(typename S::Word* __restri
                 const float d = dj[jj];
                 if (pix[jj] >= 0 && d >= min_d && d <= max_d) {
                     const float sdf = d - cz[jj];
-                    if (!(sdf < -trunc)) tn = smin(1.0f, sdf / trunc);
+                    if (!(sdf < -trunc)) tn = smin(1.0f, divBy<FD>(dtr, sdf));
                 }
                 L[lane * kLdsStride + j0 + jj] = tn;
             }
@@ -354,6 +357,72 @@ __global__ __launch_bounds__(256) void integrateFast(typename S::Word* __restric
     }
 }
 
+// Exhaustive comparison of a fast form with the IEEE result over every float whose
+// magnitude's bit pattern lies in [lo, hi), both signs; counts the mismatches (one
+// atomic per wave that found any). mode 0: divFixed(x) vs x / b; mode 1: recipNR(x) vs
+// 1.0f / x.
+__global__ void verifyKernel(int mode, float b, float r, unsigned lo, unsigned hi, unsigned* __restrict__ bad) {
+    const FixedDiv d{b, r, 1};
+    const unsigned long long n = 2ull * (unsigned long long)(hi - lo);
+    unsigned miss = 0;
+    for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (unsigned long long)gridDim.x * blockDim.x) {
+        const unsigned bits = (lo + (unsigned)(i >> 1)) | ((unsigned)(i & 1) << 31);
+        const float x = __uint_as_float(bits);
+        const float fast = mode == 0 ? divFixed(d, x) : recipNR(x);
+        const float ieee = mode == 0 ? x / b : 1.0f / x;
+        miss += !(fast == ieee);
+    }
+    const unsigned long long any = __ballot(miss != 0);
+    if (any) {
+        unsigned total = miss;
+        for (int o = 32; o > 0; o >>= 1) total += __shfl_xor(total, o);
+        if ((threadIdx.x & 63) == 0) atomicAdd(bad, total);
+    }
+}
+
+static unsigned runCheck(int mode, float b, float r, unsigned lo, unsigned hi) {
+    unsigned* bad = nullptr;
+    unsigned h = 1;
+    if (hipMalloc(&bad, sizeof(unsigned)) != hipSuccess) {
+        (void)hipGetLastError();
+        return 1;
+    }
+    bool ok = hipMemset(bad, 0, sizeof(unsigned)) == hipSuccess;
+    if (ok) {
+        hipLaunchKernelGGL(verifyKernel, dim3(8192), dim3(256), 0, nullptr, mode, b, r, lo, hi, bad);
+        ok = hipGetLastError() == hipSuccess && hipMemcpy(&h, bad, sizeof(unsigned), hipMemcpyDeviceToHost) == hipSuccess;
+    }
+    (void)hipFree(bad);
+    (void)hipGetLastError();
+    return ok ? h : 1;
+}
+
+bool recipVerified() {
+    static std::mutex mu;
+    static std::map<int, bool> known;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = known.find(dev);
+    if (it != known.end()) return it->second;
+    // 2^-126 <= |x| < 2^64: bit patterns [0x00800000, 0x5F800000)
+    const bool ok = runCheck(1, 0.0f, 0.0f, 0x00800000u, 0x5F800000u) == 0;
+    known[dev] = ok;
+    return ok;
+}
+
+FixedDiv makeFixedDiv(float b) {
+    FixedDiv d{b, 1.0f / b, 0};
+    if (!(b > 0.0f) || !std::isfinite(d.r)) return d;
+    // 2^-100 <= |x| < 2^32: bit patterns [0x0D800000, 0x4F800000)
+    d.fast = runCheck(0, b, d.r, 0x0D800000u, 0x4F800000u) == 0;
+    return d;
+}
+
 // Reset to the empty state.
 template <class S>
 __global__ void fillEmpty(typename S::Word* __restrict__ vox, size_t n) {
@@ -365,20 +434,33 @@ __global__ void fillEmpty(typename S::Word* __restrict__ vox, size_t n) {
 // ---------------------------------------------------------------------------
 // Host launchers (called from capi.hip).
 // ---------------------------------------------------------------------------
+// dtr: division by the truncation, verified exact (dtr.fast) or IEEE; recip: 1 / c.z by
+// the verified Newton-Raphson reciprocal.
 hipError_t launchIntegrate(int format, bool exact, void* vox, const VolParams& p, Intr K, int W, int H,
-                           const float* depth, float min_d, float max_d, const PoseArg& pa,
-                           unsigned long long* counters, hipStream_t st) {
+                           const float* depth, float min_d, float max_d, const PoseArg& pa, const FixedDiv& dtr,
+                           bool recip, unsigned long long* counters, hipStream_t st) {
     if (exact) {
         // One wave per (y, 64-row z group) of the whole volume; waves outside the
         // frustum box exit at once (the box is only known on the device).
         const long long tasks = (long long)p.res * ((p.res + 63) / 64);
-        const unsigned blocks = (unsigned)((tasks + 3) / 4);
-        if (format == kFmtF32)
-            hipLaunchKernelGGL(integrateExact<StoreF32>, dim3(blocks), dim3(256), 0, st, (float2*)vox, p, K, W, H,
-                               depth, min_d, max_d, pa, counters);
-        else
-            hipLaunchKernelGGL(integrateExact<StoreF16>, dim3(blocks), dim3(256), 0, st, (uint32_t*)vox, p, K, W,
-                               H, depth, min_d, max_d, pa, counters);
+        const dim3 grid((unsigned)((tasks + 3) / 4)), block(256);
+        const bool fd = dtr.fast != 0, rc = recip;
+#define AZU_INT_EXACT(S, W, FD, RC)                                                                        \
+    hipLaunchKernelGGL((integrateExact<S, FD, RC>), grid, block, 0, st, (W*)vox, p, K, W_, H, depth, min_d, \
+                       max_d, pa, dtr, counters)
+        const int W_ = W;
+        if (format == kFmtF32) {
+            if (fd && rc) AZU_INT_EXACT(StoreF32, float2, true, true);
+            else if (fd) AZU_INT_EXACT(StoreF32, float2, true, false);
+            else if (rc) AZU_INT_EXACT(StoreF32, float2, false, true);
+            else AZU_INT_EXACT(StoreF32, float2, false, false);
+        } else {
+            if (fd && rc) AZU_INT_EXACT(StoreF16, uint32_t, true, true);
+            else if (fd) AZU_INT_EXACT(StoreF16, uint32_t, true, false);
+            else if (rc) AZU_INT_EXACT(StoreF16, uint32_t, false, true);
+            else AZU_INT_EXACT(StoreF16, uint32_t, false, false);
+        }
+#undef AZU_INT_EXACT
     } else {
         const dim3 grid((p.res + 63) / 64, (p.res + 15) / 16, (p.res + kFastZ - 1) / kFastZ);
         if (format == kFmtF32)

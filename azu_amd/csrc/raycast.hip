@@ -51,6 +51,7 @@ template <class S> struct Vol {
     int res;
     float vs, inv_vs;
     float ox, oy, oz;
+    FixedDiv dvs;  // division by vs
 };
 
 // sampleTSDF (TSDFVolume.cpp:492-525) in two halves, so a ray can issue the reads of
@@ -63,14 +64,14 @@ template <class S> struct Cell {
     bool inside;
 };
 
-template <class S, bool E>
+template <class S, bool E, bool FD>
 __device__ __forceinline__ Cell<S> sampleLoad(const Vol<S>& V, F3 p) {
     Cell<S> c;
     float gx, gy, gz;
     if (E) {
-        gx = (p.x - V.ox) / V.vs;
-        gy = (p.y - V.oy) / V.vs;
-        gz = (p.z - V.oz) / V.vs;
+        gx = divBy<FD>(V.dvs, p.x - V.ox);
+        gy = divBy<FD>(V.dvs, p.y - V.oy);
+        gz = divBy<FD>(V.dvs, p.z - V.oz);
     } else {
         gx = (p.x - V.ox) * V.inv_vs;
         gy = (p.y - V.oy) * V.inv_vs;
@@ -96,10 +97,7 @@ __device__ __forceinline__ Cell<S> sampleLoad(const Vol<S>& V, F3 p) {
 template <class S, bool E>
 __device__ __forceinline__ bool sampleEval(const Cell<S>& c, float& f) {
     if (!c.inside) return false;
-    if (!(S::weight(c.q0.a) > 0.0f && S::weight(c.q0.b) > 0.0f && S::weight(c.q1.a) > 0.0f &&
-          S::weight(c.q1.b) > 0.0f && S::weight(c.q2.a) > 0.0f && S::weight(c.q2.b) > 0.0f &&
-          S::weight(c.q3.a) > 0.0f && S::weight(c.q3.b) > 0.0f))
-        return false;
+    if (!S::observed8(c.q0.a, c.q0.b, c.q1.a, c.q1.b, c.q2.a, c.q2.b, c.q3.a, c.q3.b)) return false;
     const float tx = c.tx, ty = c.ty, tz = c.tz;
     const float c0 = S::tsdf(c.q0.a), c1 = S::tsdf(c.q0.b), c2 = S::tsdf(c.q1.a), c3 = S::tsdf(c.q1.b);
     const float c4 = S::tsdf(c.q2.a), c5 = S::tsdf(c.q2.b), c6 = S::tsdf(c.q3.a), c7 = S::tsdf(c.q3.b);
@@ -120,13 +118,13 @@ __device__ __forceinline__ bool sampleEval(const Cell<S>& c, float& f) {
     return true;
 }
 
-template <class S, bool E>
+template <class S, bool E, bool FD>
 __device__ __forceinline__ bool sampleVol(const Vol<S>& V, F3 p, float& f) {
     float gx, gy, gz;
     if (E) {
-        gx = (p.x - V.ox) / V.vs;
-        gy = (p.y - V.oy) / V.vs;
-        gz = (p.z - V.oz) / V.vs;
+        gx = divBy<FD>(V.dvs, p.x - V.ox);
+        gy = divBy<FD>(V.dvs, p.y - V.oy);
+        gz = divBy<FD>(V.dvs, p.z - V.oz);
     } else {
         gx = (p.x - V.ox) * V.inv_vs;
         gy = (p.y - V.oy) * V.inv_vs;
@@ -151,16 +149,16 @@ __device__ __forceinline__ bool sampleVol(const Vol<S>& V, F3 p, float& f) {
 }
 
 // computeNormal (TSDFVolume.cpp:478-490)
-template <class S, bool E>
+template <class S, bool E, bool FD>
 __device__ __forceinline__ bool normalVol(const Vol<S>& V, F3 p, F3& n) {
     const float h = V.vs;
     float s0, s1, s2, s3, s4, s5;
-    if (!sampleVol<S, E>(V, F3{p.x + h, p.y + 0.0f, p.z + 0.0f}, s0) ||
-        !sampleVol<S, E>(V, F3{p.x - h, p.y - 0.0f, p.z - 0.0f}, s1) ||
-        !sampleVol<S, E>(V, F3{p.x + 0.0f, p.y + h, p.z + 0.0f}, s2) ||
-        !sampleVol<S, E>(V, F3{p.x - 0.0f, p.y - h, p.z - 0.0f}, s3) ||
-        !sampleVol<S, E>(V, F3{p.x + 0.0f, p.y + 0.0f, p.z + h}, s4) ||
-        !sampleVol<S, E>(V, F3{p.x - 0.0f, p.y - 0.0f, p.z - h}, s5))
+    if (!sampleVol<S, E, FD>(V, F3{p.x + h, p.y + 0.0f, p.z + 0.0f}, s0) ||
+        !sampleVol<S, E, FD>(V, F3{p.x - h, p.y - 0.0f, p.z - 0.0f}, s1) ||
+        !sampleVol<S, E, FD>(V, F3{p.x + 0.0f, p.y + h, p.z + 0.0f}, s2) ||
+        !sampleVol<S, E, FD>(V, F3{p.x - 0.0f, p.y - h, p.z - 0.0f}, s3) ||
+        !sampleVol<S, E, FD>(V, F3{p.x + 0.0f, p.y + 0.0f, p.z + h}, s4) ||
+        !sampleVol<S, E, FD>(V, F3{p.x - 0.0f, p.y - 0.0f, p.z - h}, s5))
         return false;
     const F3 g{s0 - s1, s2 - s3, s4 - s5};
     const float len = f3norm(g);
@@ -178,7 +176,7 @@ struct RayOut {
 // Far-step samples whose reads one ray issues together, per storage form.
 template <class S> constexpr int kSpecOf = S::kBytes == 8 ? RAY_SPEC32 : RAY_SPEC16;
 
-template <class S, bool E>
+template <class S, bool E, bool FD>
 __global__ __launch_bounds__(256) void raycastKernel(Vol<S> V, VolParams p, PoseArg pa, Intr K, int W, int H,
                                                      RayOut out) {
     // XCD-aware tile order: XCD k (blocks b with b % 8 == k) gets a contiguous band.
@@ -231,7 +229,7 @@ __global__ __launch_bounds__(256) void raycastKernel(Vol<S> V, VolParams p, Pose
     F3 hit{0, 0, 0}, nrm{0, 0, 0};
     if (t_near < t_far) {
         float t_prev = t_near, f_prev = 1.0f;
-        bool prev_valid = sampleVol<S, E>(V, f3add(cam, f3mul(dir, t_prev)), f_prev);
+        bool prev_valid = sampleVol<S, E, FD>(V, f3add(cam, f3mul(dir, t_prev)), f_prev);
         bool go = !(prev_valid && (!isfinite(f_prev) || f_prev < 0.0f));
         while (go && t_prev < t_far) {
             if (kSpecOf<S> > 0 && (!prev_valid || f_prev >= 0.999f)) {
@@ -250,7 +248,7 @@ __global__ __launch_bounds__(256) void raycastKernel(Vol<S> V, VolParams p, Pose
                     ts[k] = tt;
                 }
 #pragma unroll
-                for (int k = 0; k < kSpec; ++k) cs[k] = sampleLoad<S, E>(V, f3add(cam, f3mul(dir, ts[k])));
+                for (int k = 0; k < kSpec; ++k) cs[k] = sampleLoad<S, E, FD>(V, f3add(cam, f3mul(dir, ts[k])));
 #pragma unroll
                 for (int k = 0; k < kSpec; ++k) {
                     const float t = ts[k];
@@ -290,7 +288,7 @@ __global__ __launch_bounds__(256) void raycastKernel(Vol<S> V, VolParams p, Pose
                 (!prev_valid || f_prev >= 0.999f) ? far_step : smax(min_step, 0.8f * f_prev * trunc);
             const float t = smin(t_prev + step, t_far);
             float f = 1.0f;
-            const bool valid = sampleVol<S, E>(V, f3add(cam, f3mul(dir, t)), f);
+            const bool valid = sampleVol<S, E, FD>(V, f3add(cam, f3mul(dir, t)), f);
             if (valid && !isfinite(f)) break;
             if (valid && prev_valid) {
                 if (f_prev > 0.0f && f <= 0.0f) {
@@ -309,7 +307,7 @@ __global__ __launch_bounds__(256) void raycastKernel(Vol<S> V, VolParams p, Pose
         }
         if (found) {
             hit = f3add(cam, f3mul(dir, t_hit));
-            found = normalVol<S, E>(V, hit, nrm) && f3finite(hit);
+            found = normalVol<S, E, FD>(V, hit, nrm) && f3finite(hit);
         }
     }
     if (!found) {
@@ -328,7 +326,7 @@ __global__ __launch_bounds__(256) void raycastKernel(Vol<S> V, VolParams p, Pose
 }
 
 template <class S>
-Vol<S> makeVol(const void* vox, const VolParams& p) {
+Vol<S> makeVol(const void* vox, const VolParams& p, const FixedDiv& dvs) {
     Vol<S> V;
     V.vox = static_cast<const typename S::Word*>(vox);
     V.res = p.res;
@@ -337,30 +335,35 @@ Vol<S> makeVol(const void* vox, const VolParams& p) {
     V.ox = p.origin[0];
     V.oy = p.origin[1];
     V.oz = p.origin[2];
+    V.dvs = dvs;
     return V;
 }
 
-hipError_t launchRaycast(int format, bool exact, const void* vox, const VolParams& p, const PoseArg& pa, Intr K,
-                         int W, int H, float* verts, float* norms, float* t_hit, hipStream_t st) {
+template <class S, bool E, bool FD>
+void launchRay(const void* vox, const VolParams& p, const FixedDiv& dvs, const PoseArg& pa, Intr K, int W, int H,
+               const RayOut& out, hipStream_t st) {
     const int nt = ((W + 15) / 16) * ((H + 15) / 16);
     const unsigned blocks = (unsigned)(((nt + 7) / 8) * 8);
+    hipLaunchKernelGGL((raycastKernel<S, E, FD>), dim3(blocks), dim3(256), 0, st, makeVol<S>(vox, p, dvs), p, pa, K,
+                       W, H, out);
+}
+
+// dvs: division by the voxel size, verified exact (dvs.fast) or IEEE.
+hipError_t launchRaycast(int format, bool exact, const void* vox, const VolParams& p, const FixedDiv& dvs,
+                         const PoseArg& pa, Intr K, int W, int H, float* verts, float* norms, float* t_hit,
+                         hipStream_t st) {
     const RayOut out{verts, norms, t_hit};
+    const bool fd = dvs.fast != 0;
     if (format == kFmtF32) {
-        if (exact)
-            hipLaunchKernelGGL((raycastKernel<StoreF32, true>), dim3(blocks), dim3(256), 0, st, makeVol<StoreF32>(vox, p),
-                               p, pa, K, W, H, out);
-        else
-            hipLaunchKernelGGL((raycastKernel<StoreF32, false>), dim3(blocks), dim3(256), 0, st,
-                               makeVol<StoreF32>(vox, p), p, pa, K, W, H, out);
+        if (!exact) launchRay<StoreF32, false, false>(vox, p, dvs, pa, K, W, H, out, st);
+        else if (fd) launchRay<StoreF32, true, true>(vox, p, dvs, pa, K, W, H, out, st);
+        else launchRay<StoreF32, true, false>(vox, p, dvs, pa, K, W, H, out, st);
     } else {
-        if (exact)
-            hipLaunchKernelGGL((raycastKernel<StoreF16, true>), dim3(blocks), dim3(256), 0, st,
-                               makeVol<StoreF16>(vox, p), p, pa, K, W, H, out);
-        else
-            hipLaunchKernelGGL((raycastKernel<StoreF16, false>), dim3(blocks), dim3(256), 0, st,
-                               makeVol<StoreF16>(vox, p), p, pa, K, W, H, out);
+        // The FP16 march is instruction-bound rather than latency-bound, and there the
+        // guarded reciprocal division measures slower than the IEEE one: not used.
+        if (!exact) launchRay<StoreF16, false, false>(vox, p, dvs, pa, K, W, H, out, st);
+        else launchRay<StoreF16, true, false>(vox, p, dvs, pa, K, W, H, out, st);
     }
     return hipGetLastError();
 }
-
 }  // namespace azu

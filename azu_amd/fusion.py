@@ -72,6 +72,13 @@ class TSDFVolume:
         check(lib.azu_vol_set_params(self.h, ctypes.byref(cp)))
         self.params = params
 
+    def division_check(self):
+        """(voxel_size, truncation, reciprocal): True where the exact kernels use the
+        verified fast form instead of the IEEE division."""
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(lib.azu_vol_division_check(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return bool(a.value), bool(b.value), bool(c.value)
+
     def reset(self, stream=None):
         check(lib.azu_vol_reset(self.h, stream))
 

@@ -70,6 +70,7 @@ def run_form(frames_dev, nframes, pose0, params, fmt, W, H, K, warmup, steps):
     from azu_amd.pipeline import FrameLoop
 
     loop = FrameLoop(params, fmt, W, H, K)
+    log("verified fixed-divisor division (voxel size, truncation):", loop.vol.division_check())
     loop.reset(pose0)
     for f in range(warmup):
         loop.frame(frames_dev[f])

@@ -121,6 +121,14 @@ int azu_vol_reset(azu_volume* v, azu_stream s);
 int azu_vol_set_params(azu_volume* v, const azu_tsdf_params* p);
 int azu_vol_get_params(const azu_volume* v, azu_tsdf_params* p, int* format);
 int azu_vol_set_mode(azu_volume* v, int mode);
+/* Diagnostic: whether the EXACT kernels divide by the voxel size / the truncation with
+ * a reciprocal and two exact corrections (1) or with an IEEE division (0), and take the
+ * camera depth's reciprocal by Newton-Raphson (1) or IEEE (0). Each fast form is chosen
+ * only when an exhaustive device check (at create / setParams; once per device for the
+ * reciprocal) found it equal to the IEEE result over the inputs that matter (every float
+ * |x| < 2^32; every finite float); results are the same either way. Any pointer may be
+ * NULL. No counterpart in the reference. */
+int azu_vol_division_check(const azu_volume* v, int* voxel_size_fast, int* truncation_fast, int* reciprocal_fast);
 /* Whole-volume copies to / from host float arrays of res^3 (voxelData / voxelAt).
  * Synchronise. */
 int azu_vol_download(azu_volume* v, float* tsdf, float* weight);

@@ -26,8 +26,8 @@ def test_device_present():
 @pytest.mark.parametrize("res,vs", [(64, 0.016), (128, 0.008), (256, 0.004)])
 def test_integrate_exact_bitwise(res, vs):
     p = small_params(res, vs)
-    say("integrate exact", res)
     g = gpu_volume(p)
+    say("integrate exact", res, "verified fixed-divisor division (voxel size, truncation):", g.division_check())
     o = oracle_volume(p)
     upd_total = 0
     for mm, T in frames(3):
