@@ -155,7 +155,7 @@ __global__ __launch_bounds__(256) void integrateExact(typename S::Word* __restri
     float* L = lds[wv];
     const size_t res = (size_t)p.res, plane = res * res;
     const float fW = (float)W, fH = (float)H;
-    unsigned long long n_upd = 0, n_new = 0;  // wave-uniform
+    unsigned n_upd = 0, n_new = 0;  // per lane, summed over the wave at the end
 
     for (int xc = f.x0; xc <= f.x1; xc += kChunk) {
         const int n = min(kChunk, f.x1 - xc + 1);
@@ -229,13 +229,21 @@ __global__ __launch_bounds__(256) void integrateExact(typename S::Word* __restri
                     col[(size_t)(zb + g + 2 * (k0 + k)) * plane] =
                         S::pack((t_old * w_old + tn[k]) / (w_old + 1.0f), smin(w_old + 1.0f, p.max_w));
                 }
-                n_upd += (unsigned long long)__popcll(__ballot(upd));
-                n_new += (unsigned long long)__popcll(__ballot(newly));
+                n_upd += upd ? 1u : 0u;
+                n_new += newly ? 1u : 0u;
             }
         }
         waveSync();
     }
-    flushCounts(counters, n_upd, n_new);
+    if (counters) {
+        unsigned long long a = n_upd, b = n_new;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            a += __shfl_xor(a, o);
+            b += __shfl_xor(b, o);
+        }
+        flushCounts(counters, a, b);
+    }
 }
 
 // ---------------------------------------------------------------------------
