@@ -19,7 +19,8 @@
 namespace azu {
 hipError_t launchIntegrate(int format, bool exact, void* vox, const VolParams& p, Intr K, int W, int H,
                            const float* depth, float min_d, float max_d, const PoseArg& pa, const FixedDiv& dtr,
-                           bool recip, unsigned long long* counters, hipStream_t st);
+                           bool recip, void* bounds, unsigned long long* counters, hipStream_t st);
+size_t depthBoundsBytes(int W, int H);
 hipError_t launchFillEmpty(int format, void* vox, size_t n, hipStream_t st);
 hipError_t launchRaycast(int format, bool exact, const void* vox, const VolParams& p, const FixedDiv& dvs,
                          const PoseArg& pa, Intr K, int W, int H, float* verts, float* norms, float* t_hit,
@@ -204,6 +205,8 @@ struct azu_volume {
     int frames = 0;
     FixedDiv dvs{1.0f, 1.0f, 0}, dtr{1.0f, 1.0f, 0};  // division by voxel size / truncation
     bool recip = false;                                // verified Newton-Raphson 1 / x
+    void* bounds = nullptr;                            // depth bounds of the last frame size
+    size_t bounds_bytes = 0;
     bool exact() const { return mode != AZU_MODE_FAST; }
 };
 
@@ -356,6 +359,7 @@ int azu_vol_destroy(azu_volume* v) {
     if (v->counters) (void)hipFree(v->counters);
     if (v->adj_partials) (void)hipFree(v->adj_partials);
     if (v->adj_out) (void)hipFree(v->adj_out);
+    if (v->bounds) (void)hipFree(v->bounds);
     delete v;
     return AZU_OK;
 }
@@ -496,8 +500,22 @@ static int integrateImpl(azu_volume* v, const float* d_depth, int W, int H, cons
     ProfScope ps(exact ? (v->format == AZU_FMT_F32 ? "integrate_exact_f32" : "integrate_exact_f16")
                        : (v->format == AZU_FMT_F32 ? "integrate_fast_f32" : "integrate_fast_f16"),
                  (hipStream_t)s);
+    if (exact) {
+        const size_t need = depthBoundsBytes(W, H);
+        if (need > v->bounds_bytes) {
+            // grows only when the frame size does; the old buffer may still be read by
+            // work queued on another stream, so wait for the device first
+            AZU_HIP_TRY(hipDeviceSynchronize(), "synchronise before depth bounds reallocation");
+            if (v->bounds) (void)hipFree(v->bounds);
+            v->bounds = nullptr;
+            v->bounds_bytes = 0;
+            if (hipMalloc(&v->bounds, need) != hipSuccess) return fail(AZU_ERR_ALLOC, "depth bounds allocation failed");
+            v->bounds_bytes = need;
+        }
+    }
     AZU_HIP_TRY(launchIntegrate(v->format, exact, v->vox, toVol(v->params), Intr{K[0], K[1], K[2], K[3]}, W, H,
-                                d_depth, min_d, max_d, pa, v->dtr, v->recip, v->counters, (hipStream_t)s),
+                                d_depth, min_d, max_d, pa, v->dtr, v->recip, exact ? v->bounds : nullptr, v->counters,
+                                (hipStream_t)s),
                 "integrate launch");
     v->frames += 1;
     return AZU_OK;

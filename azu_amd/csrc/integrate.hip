@@ -94,19 +94,100 @@ constexpr int kChunk = 32;  // x columns per tile
 constexpr int kSub = 16;    // columns whose depth gathers are issued together
 constexpr int kLdsStride = kChunk + 1;
 
-// Conservative, wave-uniform cull of the tile {xa..xb} x {y} x {za..zb} (a rectangle of
-// lattice points at fixed y): false only when no voxel of it can pass the gates, i.e.
-// the whole rectangle is behind the camera, beyond max_depth + trunc, or projects
-// wholly outside one image edge. The rectangle's camera coordinates are affine in
-// (x, z) and the perspective image of a rectangle in front of the camera is the convex
-// hull of its corners' images, so testing the four corners suffices. The margins
-// (1 mm in depth, 2 px in the image, corners at least 10 cm in front) cover the
-// rounding of the incremental x-step the exact kernel uses for each voxel (below
-// 1e-4 m after 2048 steps), so a culled voxel is one the exact path would reject.
-__device__ __forceinline__ bool tileMayUpdate(const IntFrame& f, const VolParams& p, Intr K, int W, int H,
-                                              float max_d, int xa, int xb, int y, int za, int zb) {
+// Depth bounds of the frame: per cell of 2^l x 2^l pixels (l = 1..kBoundLevels), the
+// smallest and largest in-band depth {dmin, dmax}; dmin = -1 when any pixel of the cell
+// is outside the band (or outside the image), dmax = -inf when none is inside. The
+// integration uses them to decide whole tiles without projecting every voxel.
+constexpr int kBoundLevels = 6;
+struct DepthBounds {
+    const float2* lvl;              // all levels, level l at lvl + off[l]
+    int off[kBoundLevels + 1];
+    int w[kBoundLevels + 1];        // cells per row
+    int h[kBoundLevels + 1];
+};
+
+__device__ __forceinline__ float2 boundsJoin(float2 a, float2 b) {
+    return make_float2((a.x < 0.0f || b.x < 0.0f) ? -1.0f : fminf(a.x, b.x), fmaxf(a.y, b.y));
+}
+
+// One block per 64 x 64 pixel region: levels 1..6 of its cells, reduced in LDS.
+__global__ __launch_bounds__(256) void depthBoundsKernel(const float* __restrict__ depth, int W, int H, float min_d,
+                                                         float max_d, float2* __restrict__ out, DepthBounds b) {
+    __shared__ float2 cell[32][33];
+    const int tid = threadIdx.x;
+    const int X0 = blockIdx.x * 64, Y0 = blockIdx.y * 64;
+    // level 1: 32 x 32 cells of 2 x 2 pixels, 4 per thread
+    for (int k = tid; k < 32 * 32; k += 256) {
+        const int cx = k & 31, cy = k >> 5;
+        float2 acc = make_float2(__builtin_inff(), -__builtin_inff());
+        for (int j = 0; j < 4; ++j) {
+            const int x = X0 + 2 * cx + (j & 1), y = Y0 + 2 * cy + (j >> 1);
+            float2 pxb = make_float2(-1.0f, -__builtin_inff());
+            if (x < W && y < H) {
+                const float d = depth[(size_t)y * W + x];
+                if (d >= min_d && d <= max_d) pxb = make_float2(d, d);
+            }
+            acc = j == 0 ? pxb : boundsJoin(acc, pxb);
+        }
+        cell[cy][cx] = acc;
+        const int gx = blockIdx.x * 32 + cx, gy = blockIdx.y * 32 + cy;
+        if (gx < b.w[1] && gy < b.h[1]) out[b.off[1] + gy * b.w[1] + gx] = acc;
+    }
+    __syncthreads();
+    for (int l = 2, n = 16; l <= kBoundLevels; ++l, n >>= 1) {
+        float2 v = make_float2(0.0f, 0.0f);
+        const bool act = tid < n * n;
+        const int cx = tid % n, cy = tid / n;
+        if (act)
+            v = boundsJoin(boundsJoin(cell[2 * cy][2 * cx], cell[2 * cy][2 * cx + 1]),
+                           boundsJoin(cell[2 * cy + 1][2 * cx], cell[2 * cy + 1][2 * cx + 1]));
+        __syncthreads();
+        if (act) {
+            cell[cy][cx] = v;
+            const int gx = blockIdx.x * n + cx, gy = blockIdx.y * n + cy;
+            if (gx < b.w[l] && gy < b.h[l]) out[b.off[l] + gy * b.w[l] + gx] = v;
+        }
+        __syncthreads();
+    }
+}
+
+// Bounds over the pixel rectangle [u0, u1] x [v0, v1] (inside the image), from the
+// coarsest level needed to cover it with at most 2 x 2 cells; false when the rectangle
+// is wider than a level-kBoundLevels cell.
+__device__ __forceinline__ bool boundsOver(const DepthBounds& b, int u0, int u1, int v0, int v1, float2& r) {
+    const int ext = max(u1 - u0, v1 - v0) + 1;
+    int l = 1;
+    while ((1 << l) < ext) ++l;
+    if (l > kBoundLevels) return false;
+    const int cx0 = u0 >> l, cx1 = u1 >> l, cy0 = v0 >> l, cy1 = v1 >> l;
+    const float2* L = b.lvl + b.off[l];
+    r = boundsJoin(boundsJoin(L[cy0 * b.w[l] + cx0], L[cy0 * b.w[l] + cx1]),
+                   boundsJoin(L[cy1 * b.w[l] + cx0], L[cy1 * b.w[l] + cx1]));
+    return true;
+}
+
+// Wave-uniform classification of the tile {xa..xb} x {y} x {za..zb} (a rectangle of
+// lattice points at fixed y), decided from its four corners: the rectangle's camera
+// coordinates are affine in (x, z), and the perspective image of a rectangle in front of
+// the camera is the convex hull of its corners' images. The voxels' exact coordinates
+// (the incremental x-step) differ from the affine ones by at most eps (the rounding of
+// every step so far, bounded from the coordinates' magnitude), and eps is carried into
+// every margin, so the class is the outcome the exact per-voxel path would reach:
+//   kTileCull: no voxel can update (behind the camera, beyond max_depth + trunc, wholly
+//              outside one image edge, or behind every in-band depth of its footprint by
+//              more than trunc);
+//   kTileFree: every voxel projects inside the image onto an in-band pixel at least
+//              trunc + margin beyond it, so it folds tsdf_new = 1 (sdf >= trunc gives
+//              sdf / trunc >= 1);
+//   kTileExact: anything else, decided voxel by voxel.
+constexpr int kTileCull = 0, kTileExact = 1, kTileFree = 2;
+
+__device__ __forceinline__ int tileClass(const IntFrame& f, const VolParams& p, Intr K, int W, int H, float max_d,
+                                         int xa, int xb, int y, int za, int zb, const DepthBounds& bnd,
+                                         float cbound) {
     float zmin = __builtin_inff(), zmax = -__builtin_inff();
-    bool left = true, right = true, top = true, bottom = true;
+    float umin = __builtin_inff(), umax = -__builtin_inff(), vmin = __builtin_inff(), vmax = -__builtin_inff();
+    bool left = true, right = true, top = true, bottom = true, front = true;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const float px = p.origin[0] + (float)((i & 1) ? xb : xa) * p.vs;
@@ -122,19 +203,40 @@ __device__ __forceinline__ bool tileMayUpdate(const IntFrame& f, const VolParams
             right = right && u > (float)W + 2.0f;
             top = top && v < -2.0f;
             bottom = bottom && v > (float)H + 2.0f;
+            umin = fminf(umin, u);
+            umax = fmaxf(umax, u);
+            vmin = fminf(vmin, v);
+            vmax = fmaxf(vmax, v);
         } else {
-            left = right = top = bottom = false;
+            left = right = top = bottom = front = false;
         }
     }
-    if (zmax < -1e-3f) return false;
-    if (zmin > max_d + p.trunc + 1e-3f) return false;
-    return !(left || right || top || bottom);
+    if (zmax < -1e-3f) return kTileCull;
+    if (zmin > max_d + p.trunc + 1e-3f) return kTileCull;
+    if (left || right || top || bottom) return kTileCull;
+    if (!front || !bnd.lvl) return kTileExact;
+    // eps: a float add rounds by at most 2^-24 of a magnitude <= cbound, once per step
+    // of the row so far and a few times in its start point.
+    const float eps = 1.2e-7f * cbound * (float)(xb - f.x0 + 16) + 1e-5f;
+    if (!(zmin - eps > 0.1f)) return kTileExact;
+    const float mpx = 2.0f + eps / (zmin - eps) * (fmaxf(K.fx, K.fy) + (float)max(W, H) + 4.0f);
+    const float u0f = floorf(umin - mpx), u1f = floorf(umax + mpx), v0f = floorf(vmin - mpx), v1f = floorf(vmax + mpx);
+    const bool inside = u0f >= 0.0f && v0f >= 0.0f && u1f < (float)W && v1f < (float)H;
+    const int u0 = (int)fmaxf(u0f, 0.0f), v0 = (int)fmaxf(v0f, 0.0f);
+    const int u1 = (int)fminf(u1f, (float)(W - 1)), v1 = (int)fminf(v1f, (float)(H - 1));
+    if (u0 > u1 || v0 > v1) return kTileExact;
+    float2 r;
+    if (!boundsOver(bnd, u0, u1, v0, v1, r)) return kTileExact;
+    const float gap = p.trunc + 2.0f * eps + 1e-4f;
+    if (r.y + gap < zmin) return kTileCull;                      // behind every in-band depth
+    if (inside && r.x >= 0.0f && r.x - gap > zmax) return kTileFree;  // in front of every depth, all in band
+    return kTileExact;
 }
 
 template <class S, bool FD, bool RC>
 __global__ __launch_bounds__(256) void integrateExact(typename S::Word* __restrict__ vox, VolParams p, Intr K,
                                                       int W, int H, const float* __restrict__ depth, float min_d,
-                                                      float max_d, PoseArg pa, FixedDiv dtr,
+                                                      float max_d, PoseArg pa, FixedDiv dtr, DepthBounds bnd,
                                                       unsigned long long* __restrict__ counters) {
     __shared__ float lds[4][64 * kLdsStride];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -156,18 +258,22 @@ __global__ __launch_bounds__(256) void integrateExact(typename S::Word* __restri
     const size_t res = (size_t)p.res, plane = res * res;
     const float fW = (float)W, fH = (float)H;
     unsigned n_upd = 0, n_new = 0;  // per lane, summed over the wave at the end
+    // A bound on every camera coordinate's magnitude over the volume (for tileClass).
+    const float cbound = fabsf(f.t_wc.x) + fabsf(f.t_wc.y) + fabsf(f.t_wc.z) +
+                         fabsf(p.origin[0]) + fabsf(p.origin[1]) + fabsf(p.origin[2]) + 3.0f * (float)p.res * vs;
 
     for (int xc = f.x0; xc <= f.x1; xc += kChunk) {
         const int n = min(kChunk, f.x1 - xc + 1);
-        if (!tileMayUpdate(f, p, K, W, H, max_d, xc, xc + n - 1, y, zb, ze)) {
-            // Nothing to fold; the row's camera coordinate still advances step by step.
+        const int cls = tileClass(f, p, K, W, H, max_d, xc, xc + n - 1, y, zb, ze, bnd, cbound);
+        if (cls != kTileExact) {
+            // The row's camera coordinate still advances step by step.
             for (int j = 0; j < n; ++j) c = f3add(c, f.step);
-            continue;
+            if (cls == kTileCull) continue;
         }
-        // Phase 1: lane = row, serial over the tile's columns. Per sub-chunk, the
-        // projections first, then all depth gathers at once, then the gates.
+        // Phase 1 (exact tiles): lane = row, serial over the tile's columns. Per
+        // sub-chunk, the projections first, then all depth gathers at once, then the gates.
 #pragma unroll 1
-        for (int j0 = 0; j0 < kChunk; j0 += kSub) {
+        for (int j0 = 0; j0 < (cls == kTileExact ? kChunk : 0); j0 += kSub) {
             float cz[kSub], dj[kSub];
             int pix[kSub];
 #pragma unroll
@@ -210,7 +316,7 @@ __global__ __launch_bounds__(256) void integrateExact(typename S::Word* __restri
 #pragma unroll
             for (int k = 0; k < 16; ++k) {
                 const int r = g + 2 * (k0 + k);
-                const float t = L[r * kLdsStride + j];
+                const float t = cls == kTileFree ? 1.0f : L[r * kLdsStride + j];
                 tn[k] = (col_ok && zb + r <= f.z1) ? t : __builtin_nanf("");
             }
             // Lanes with nothing to fold read voxel 0 instead (a cached line), so the
@@ -442,12 +548,34 @@ __global__ void fillEmpty(typename S::Word* __restrict__ vox, size_t n) {
 // ---------------------------------------------------------------------------
 // Host launchers (called from capi.hip).
 // ---------------------------------------------------------------------------
+// Bytes of the depth bounds of a W x H frame.
+size_t depthBoundsBytes(int W, int H) {
+    size_t n = 0;
+    for (int l = 1; l <= kBoundLevels; ++l)
+        n += (size_t)((W + (1 << l) - 1) >> l) * (size_t)((H + (1 << l) - 1) >> l);
+    return n * sizeof(float2);
+}
+
 // dtr: division by the truncation, verified exact (dtr.fast) or IEEE; recip: 1 / c.z by
-// the verified Newton-Raphson reciprocal.
+// the verified Newton-Raphson reciprocal; bounds: scratch for the depth bounds
+// (depthBoundsBytes), or null to decide every tile voxel by voxel.
 hipError_t launchIntegrate(int format, bool exact, void* vox, const VolParams& p, Intr K, int W, int H,
                            const float* depth, float min_d, float max_d, const PoseArg& pa, const FixedDiv& dtr,
-                           bool recip, unsigned long long* counters, hipStream_t st) {
+                           bool recip, void* bounds, unsigned long long* counters, hipStream_t st) {
     if (exact) {
+        DepthBounds bnd{};
+        if (bounds) {
+            int off = 0;
+            for (int l = 1; l <= kBoundLevels; ++l) {
+                bnd.off[l] = off;
+                bnd.w[l] = (W + (1 << l) - 1) >> l;
+                bnd.h[l] = (H + (1 << l) - 1) >> l;
+                off += bnd.w[l] * bnd.h[l];
+            }
+            bnd.lvl = static_cast<const float2*>(bounds);
+            hipLaunchKernelGGL(depthBoundsKernel, dim3((W + 63) / 64, (H + 63) / 64), dim3(256), 0, st, depth, W, H,
+                               min_d, max_d, static_cast<float2*>(bounds), bnd);
+        }
         // One wave per (y, 64-row z group) of the whole volume; waves outside the
         // frustum box exit at once (the box is only known on the device).
         const long long tasks = (long long)p.res * ((p.res + 63) / 64);
@@ -455,7 +583,7 @@ hipError_t launchIntegrate(int format, bool exact, void* vox, const VolParams& p
         const bool fd = dtr.fast != 0, rc = recip;
 #define AZU_INT_EXACT(S, W, FD, RC)                                                                        \
     hipLaunchKernelGGL((integrateExact<S, FD, RC>), grid, block, 0, st, (W*)vox, p, K, W_, H, depth, min_d, \
-                       max_d, pa, dtr, counters)
+                       max_d, pa, dtr, bnd, counters)
         const int W_ = W;
         if (format == kFmtF32) {
             if (fd && rc) AZU_INT_EXACT(StoreF32, float2, true, true);
