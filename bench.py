@@ -142,7 +142,7 @@ def profile_form(frames_dev, pose0, params, fmt, W, H, K, nframes):
     return prof, upd / nf, icp
 
 
-def rooflines(prof, nprof, n_upd, icp, s_v, W, H):
+def rooflines(prof, nprof, n_upd, icp, s_v, W, H, res):
     """Achieved HBM GB/s of each hot kernel from its algorithmic bytes per frame and its
     event-timed duration per frame (equal to bytes per launch / average launch time)."""
     out = {}
@@ -151,7 +151,9 @@ def rooflines(prof, nprof, n_upd, icp, s_v, W, H):
         if k.startswith("integrate"):
             # read + write of every updated voxel, plus the float depth frame read once
             b = 2 * s_v * n_upd + 4 * W * H
-            extra = {"updated_voxels": int(n_upd)}
+            # the same time against the bytes of a read + write of the whole volume
+            full = 2 * s_v * res ** 3 / (total_ms / launches * 1e-3) / 1e9
+            extra = {"updated_voxels": int(n_upd), "full_sweep_equiv_frac": round(full / HBM_PEAK_GBS, 4)}
         elif k == "icp_iter":
             # live vertex + normal per interior pixel, model vertex + normal per in-bounds
             # projection, and the per-block partials (34 doubles x 256 blocks, written and
@@ -195,9 +197,9 @@ def cpu_baseline(frames_mm, pose0, params, K, nframes):
     # frame 0 has no ICP; the rate is over frames 1.. (median frame time)
     med = float(np.median(ms[1:, 4])) if nframes > 1 else float(ms[0, 4])
     return {"value": 1000.0 / med, "unit": "frames/s", "cores": int(oracle.lib.orc_num_threads()), "kind": "port",
-            "sample": "frames 1..%d of the same 512^3 sequence (%.1f s wall), median frame time %.1f ms "
+            "sample": "frames 1..%d of the same %d^3 sequence (%.1f s wall), median frame time %.1f ms "
                       "(pyramid %.1f, icp %.1f, integrate %.1f, raycast %.1f ms)" % (
-                          nframes - 1, dt, med, np.median(ms[1:, 0]), np.median(ms[1:, 1]),
+                          nframes - 1, params["resolution"], dt, med, np.median(ms[1:, 0]), np.median(ms[1:, 1]),
                           np.median(ms[1:, 2]), np.median(ms[1:, 3]))}
 
 
@@ -248,7 +250,7 @@ def main():
     per_frame = {k: v[1] / nprof for k, v in prof.items()}
     dom = max(per_frame, key=per_frame.get)
     s_v = 4 if head == "f16" else 8
-    roofs = rooflines(prof, nprof, n_upd, icp, s_v, W, H)
+    roofs = rooflines(prof, nprof, n_upd, icp, s_v, W, H, w["res"])
     roof = roofs.get(dom) or {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
                               "traffic": None, "kernel": dom,
                               "avg_launch_us": round(1000 * prof[dom][1] / prof[dom][0], 2)}
@@ -280,10 +282,13 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": "512^3 TSDF at 2 mm, 640x480 uint16 depth, sphere+plane+10 boxes, orbit+jitter, "
-                        "integrate+raycast+ICP{10,5,4}",
+            "workload": "%d^3 TSDF at %g mm, %dx%d uint16 depth%s, sphere+plane%s, orbit%s, "
+                        "integrate+raycast+ICP{10,5,4}" % (
+                            w["res"], 1000 * w["vs"], W, H, " (2x bicubic upsampled)" if W != 640 else "",
+                            "+%d boxes" % w["boxes"] if w["boxes"] else "", "+jitter" if w["jitter"] else ""),
             "tsdf_storage": "fp16+u16 (4 B/voxel)" if head == "f16" else "fp32+fp32 (8 B/voxel)",
             "frames": nframes,
+            "full_sweep_bytes_per_integrate": 2 * (4 if head == "f16" else 8) * w["res"] ** 3,
             "device": dev_name,
         },
         "fp32_tsdf": ({"value": round(results["f32"]["fps"], 2), "ms_per_step": round(results["f32"]["ms_per_step"], 4),
@@ -293,7 +298,8 @@ def main():
         "max_track_err_m": r["max_track_err_m"],
         "pcie_inclusive_fps": round(pcie, 2),
         "roofline": roof,
-        "roofline_by_kernel": {k: {kk: v[kk] for kk in ("achieved", "frac", "avg_launch_us")} for k, v in roofs.items()},
+        "roofline_by_kernel": {k: {kk: v[kk] for kk in ("achieved", "frac", "avg_launch_us", "full_sweep_equiv_frac")
+                                   if kk in v} for k, v in roofs.items()},
         "kernels_us_per_frame": {k: v["us_per_frame"] for k, v in kern.items()},
         "cpu_baseline": cpu,
     }
