@@ -101,9 +101,15 @@ __device__ __forceinline__ float divFixed(const FixedDiv& d, float x) {
     return fmaf(fmaf(-q1, d.b, x), d.r, q1);
 }
 
-template <bool FD>
+// FD = 0: IEEE division. FD = 1: divFixed where verified (|x| >= 2^-100), IEEE below.
+// FD = 2: divFixed for every x, for callers whose nonzero x cannot fall below 2^-100:
+// for x = a - b of two floats with |b| >= 2^-60, |x| < 2^-100 needs |a| > 2^-61, so both
+// are multiples of 2^-84 and so is the (exact, by Sterbenz) difference: x = +0 or
+// |x| >= 2^-84. divFixed(+0) = +0 = +0 / b.
+template <int FD>
 __device__ __forceinline__ float divBy(const FixedDiv& d, float x) {
-    if (!FD) return x / d.b;
+    if (FD == 0) return x / d.b;
+    if (FD == 2) return divFixed(d, x);
     return fabsf(x) >= 0x1p-100f ? divFixed(d, x) : x / d.b;
 }
 

@@ -233,7 +233,7 @@ __device__ __forceinline__ int tileClass(const IntFrame& f, const VolParams& p, 
     return kTileExact;
 }
 
-template <class S, bool FD, bool RC>
+template <class S, int FD, bool RC>
 __global__ __launch_bounds__(256) void integrateExact(typename S::Word* __restrict__ vox, VolParams p, Intr K,
                                                       int W, int H, const float* __restrict__ depth, float min_d,
                                                       float max_d, PoseArg pa, FixedDiv dtr, DepthBounds bnd,
@@ -580,21 +580,23 @@ hipError_t launchIntegrate(int format, bool exact, void* vox, const VolParams& p
         // frustum box exit at once (the box is only known on the device).
         const long long tasks = (long long)p.res * ((p.res + 63) / 64);
         const dim3 grid((unsigned)((tasks + 3) / 4)), block(256);
-        const bool fd = dtr.fast != 0, rc = recip;
+        // sdf = d - c.z with d >= min_d: when min_d >= 2^-60 a nonzero sdf cannot fall below
+        // 2^-100 and the unguarded fixed-divisor form runs.
+        const bool fd = dtr.fast != 0, rc = recip, far = min_d >= 0x1p-60f;
 #define AZU_INT_EXACT(S, W, FD, RC)                                                                        \
     hipLaunchKernelGGL((integrateExact<S, FD, RC>), grid, block, 0, st, (W*)vox, p, K, W_, H, depth, min_d, \
                        max_d, pa, dtr, bnd, counters)
         const int W_ = W;
         if (format == kFmtF32) {
-            if (fd && rc) AZU_INT_EXACT(StoreF32, float2, true, true);
-            else if (fd) AZU_INT_EXACT(StoreF32, float2, true, false);
-            else if (rc) AZU_INT_EXACT(StoreF32, float2, false, true);
-            else AZU_INT_EXACT(StoreF32, float2, false, false);
+            if (fd && far && rc) AZU_INT_EXACT(StoreF32, float2, 2, true);
+            else if (fd && rc) AZU_INT_EXACT(StoreF32, float2, 1, true);
+            else if (rc) AZU_INT_EXACT(StoreF32, float2, 0, true);
+            else AZU_INT_EXACT(StoreF32, float2, 0, false);
         } else {
-            if (fd && rc) AZU_INT_EXACT(StoreF16, uint32_t, true, true);
-            else if (fd) AZU_INT_EXACT(StoreF16, uint32_t, true, false);
-            else if (rc) AZU_INT_EXACT(StoreF16, uint32_t, false, true);
-            else AZU_INT_EXACT(StoreF16, uint32_t, false, false);
+            if (fd && far && rc) AZU_INT_EXACT(StoreF16, uint32_t, 2, true);
+            else if (fd && rc) AZU_INT_EXACT(StoreF16, uint32_t, 1, true);
+            else if (rc) AZU_INT_EXACT(StoreF16, uint32_t, 0, true);
+            else AZU_INT_EXACT(StoreF16, uint32_t, 0, false);
         }
 #undef AZU_INT_EXACT
     } else {

@@ -64,7 +64,7 @@ template <class S> struct Cell {
     bool inside;
 };
 
-template <class S, bool E, bool FD>
+template <class S, bool E, int FD>
 __device__ __forceinline__ Cell<S> sampleLoad(const Vol<S>& V, F3 p) {
     Cell<S> c;
     float gx, gy, gz;
@@ -118,7 +118,7 @@ __device__ __forceinline__ bool sampleEval(const Cell<S>& c, float& f) {
     return true;
 }
 
-template <class S, bool E, bool FD>
+template <class S, bool E, int FD>
 __device__ __forceinline__ bool sampleVol(const Vol<S>& V, F3 p, float& f) {
     float gx, gy, gz;
     if (E) {
@@ -149,7 +149,7 @@ __device__ __forceinline__ bool sampleVol(const Vol<S>& V, F3 p, float& f) {
 }
 
 // computeNormal (TSDFVolume.cpp:478-490)
-template <class S, bool E, bool FD>
+template <class S, bool E, int FD>
 __device__ __forceinline__ bool normalVol(const Vol<S>& V, F3 p, F3& n) {
     const float h = V.vs;
     float s0, s1, s2, s3, s4, s5;
@@ -176,7 +176,7 @@ struct RayOut {
 // Far-step samples whose reads one ray issues together, per storage form.
 template <class S> constexpr int kSpecOf = S::kBytes == 8 ? RAY_SPEC32 : RAY_SPEC16;
 
-template <class S, bool E, bool FD>
+template <class S, bool E, int FD>
 __global__ __launch_bounds__(256) void raycastKernel(Vol<S> V, VolParams p, PoseArg pa, Intr K, int W, int H,
                                                      RayOut out) {
     // XCD-aware tile order: XCD k (blocks b with b % 8 == k) gets a contiguous band.
@@ -339,7 +339,7 @@ Vol<S> makeVol(const void* vox, const VolParams& p, const FixedDiv& dvs) {
     return V;
 }
 
-template <class S, bool E, bool FD>
+template <class S, bool E, int FD>
 void launchRay(const void* vox, const VolParams& p, const FixedDiv& dvs, const PoseArg& pa, Intr K, int W, int H,
                const RayOut& out, hipStream_t st) {
     const int nt = ((W + 15) / 16) * ((H + 15) / 16);
@@ -348,21 +348,27 @@ void launchRay(const void* vox, const VolParams& p, const FixedDiv& dvs, const P
                        W, H, out);
 }
 
-// dvs: division by the voxel size, verified exact (dvs.fast) or IEEE.
+// dvs: division by the voxel size, verified exact (dvs.fast) or IEEE. The divided
+// values are sample coordinates minus the volume origin; when no origin component is
+// below 2^-60 in magnitude they cannot land in (0, 2^-100) and the unguarded form runs.
 hipError_t launchRaycast(int format, bool exact, const void* vox, const VolParams& p, const FixedDiv& dvs,
                          const PoseArg& pa, Intr K, int W, int H, float* verts, float* norms, float* t_hit,
                          hipStream_t st) {
     const RayOut out{verts, norms, t_hit};
-    const bool fd = dvs.fast != 0;
+    const bool far_origin = fabsf(p.origin[0]) >= 0x1p-60f && fabsf(p.origin[1]) >= 0x1p-60f &&
+                            fabsf(p.origin[2]) >= 0x1p-60f;
+    const int fd = dvs.fast == 0 ? 0 : (far_origin ? 2 : 1);
     if (format == kFmtF32) {
-        if (!exact) launchRay<StoreF32, false, false>(vox, p, dvs, pa, K, W, H, out, st);
-        else if (fd) launchRay<StoreF32, true, true>(vox, p, dvs, pa, K, W, H, out, st);
-        else launchRay<StoreF32, true, false>(vox, p, dvs, pa, K, W, H, out, st);
+        if (!exact) launchRay<StoreF32, false, 0>(vox, p, dvs, pa, K, W, H, out, st);
+        else if (fd == 2) launchRay<StoreF32, true, 2>(vox, p, dvs, pa, K, W, H, out, st);
+        else if (fd == 1) launchRay<StoreF32, true, 1>(vox, p, dvs, pa, K, W, H, out, st);
+        else launchRay<StoreF32, true, 0>(vox, p, dvs, pa, K, W, H, out, st);
     } else {
-        // The FP16 march is instruction-bound rather than latency-bound, and there the
-        // guarded reciprocal division measures slower than the IEEE one: not used.
-        if (!exact) launchRay<StoreF16, false, false>(vox, p, dvs, pa, K, W, H, out, st);
-        else launchRay<StoreF16, true, false>(vox, p, dvs, pa, K, W, H, out, st);
+        // The FP16 march is instruction-bound: the guarded form (a branch per division)
+        // measures slower than the IEEE division there, so it takes only the unguarded one.
+        if (!exact) launchRay<StoreF16, false, 0>(vox, p, dvs, pa, K, W, H, out, st);
+        else if (fd == 2) launchRay<StoreF16, true, 2>(vox, p, dvs, pa, K, W, H, out, st);
+        else launchRay<StoreF16, true, 0>(vox, p, dvs, pa, K, W, H, out, st);
     }
     return hipGetLastError();
 }
