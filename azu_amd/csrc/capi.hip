@@ -19,12 +19,15 @@
 namespace azu {
 hipError_t launchIntegrate(int format, bool exact, void* vox, const VolParams& p, Intr K, int W, int H,
                            const float* depth, float min_d, float max_d, const PoseArg& pa, const FixedDiv& dtr,
-                           bool recip, void* bounds, unsigned long long* counters, hipStream_t st);
+                           bool recip, void* bounds, unsigned long long* counters, unsigned char* bricks,
+                           hipStream_t st);
 size_t depthBoundsBytes(int W, int H);
 hipError_t launchFillEmpty(int format, void* vox, size_t n, hipStream_t st);
+hipError_t launchBrickMap(int format, const void* vox, int res, unsigned char* bricks, bool scan, hipStream_t st);
 hipError_t launchRaycast(int format, bool exact, const void* vox, const VolParams& p, const FixedDiv& dvs,
-                         const PoseArg& pa, Intr K, int W, int H, float* verts, float* norms, float* t_hit,
-                         hipStream_t st);
+                         const unsigned char* bricks, unsigned* dsafe, int* ok, const PoseArg& pa, Intr K, int W,
+                         int H, float* verts, float* norms, float* t_hit, hipStream_t st);
+hipError_t launchSkipInit(unsigned* dsafe, size_t n, hipStream_t st);
 hipError_t launchLive(bool mm, const void* src, int W, int H, Intr K, float min_d, float max_d, float* const d[3],
                       float* const v[3], float* const n[3], hipStream_t st);
 size_t icpStateBytes();
@@ -207,6 +210,14 @@ struct azu_volume {
     bool recip = false;                                // verified Newton-Raphson 1 / x
     void* bounds = nullptr;                            // depth bounds of the last frame size
     size_t bounds_bytes = 0;
+    // Brick map for the raycast's empty-space skipping, kept by reset / upload /
+    // integrate. Once the raw storage has been handed out, other writers may exist and
+    // the map is rebuilt from the voxels before every raycast.
+    unsigned char* bricks = nullptr;
+    bool external = false;
+    bool skip = true;  // empty-space skipping in the raycast
+    unsigned* dsafe = nullptr;  // per-pixel skip bounds (+inf between raycasts), then one flag word
+    size_t dsafe_px = 0;
     bool exact() const { return mode != AZU_MODE_FAST; }
 };
 
@@ -245,6 +256,13 @@ int volAlloc(azu_volume* v) {
         v->vox = nullptr;
         return fail(AZU_ERR_ALLOC, std::string("volume allocation of ") + std::to_string(v->bytes) +
                                        " bytes failed: " + hipGetErrorString(e));
+    }
+    const size_t nb = (size_t)brickCount(v->params.resolution);
+    if (hipMalloc(&v->bricks, nb * nb * nb) != hipSuccess) {
+        v->bricks = nullptr;
+        (void)hipFree(v->vox);
+        v->vox = nullptr;
+        return fail(AZU_ERR_ALLOC, "brick map allocation failed");
     }
     return AZU_OK;
 }
@@ -334,6 +352,7 @@ int azu_vol_create(const azu_tsdf_params* p, int format, azu_volume** out) {
     }
     if (hipMalloc(&v->counters, kCounterBytes) != hipSuccess) {
         (void)hipFree(v->vox);
+        (void)hipFree(v->bricks);
         delete v;
         return fail(AZU_ERR_ALLOC, "counter allocation failed");
     }
@@ -356,6 +375,8 @@ int azu_vol_create(const azu_tsdf_params* p, int format, azu_volume** out) {
 int azu_vol_destroy(azu_volume* v) {
     if (!v) return AZU_OK;
     if (v->vox) (void)hipFree(v->vox);
+    if (v->bricks) (void)hipFree(v->bricks);
+    if (v->dsafe) (void)hipFree(v->dsafe);
     if (v->counters) (void)hipFree(v->counters);
     if (v->adj_partials) (void)hipFree(v->adj_partials);
     if (v->adj_out) (void)hipFree(v->adj_out);
@@ -370,6 +391,8 @@ int azu_vol_reset(azu_volume* v, azu_stream s) {
     {
         ProfScope ps("fill_empty", (hipStream_t)s);
         AZU_HIP_TRY(launchFillEmpty(v->format, v->vox, n, (hipStream_t)s), "fill_empty launch");
+        AZU_HIP_TRY(launchBrickMap(v->format, v->vox, v->params.resolution, v->bricks, false, (hipStream_t)s),
+                    "brick map clear");
     }
     AZU_HIP_TRY(hipMemsetAsync(v->counters, 0, kCounterBytes, (hipStream_t)s), "counter reset");
     v->frames = 0;
@@ -386,7 +409,9 @@ int azu_vol_set_params(azu_volume* v, const azu_tsdf_params* p) {
     v->params = *p;
     if (realloc_) {
         (void)hipFree(v->vox);
+        (void)hipFree(v->bricks);
         v->vox = nullptr;
+        v->bricks = nullptr;
         const int rc = volAlloc(v);
         if (rc != AZU_OK) return rc;
     }
@@ -423,6 +448,12 @@ int azu_vol_set_mode(azu_volume* v, int mode) {
     return AZU_OK;
 }
 
+int azu_vol_set_skip(azu_volume* v, int on) {
+    if (!v) return fail(AZU_ERR_ARG, "null volume");
+    v->skip = on != 0;
+    return AZU_OK;
+}
+
 static int volTransfer(azu_volume* v, float* tsdf, float* weight, bool download) {
     if (!v || !tsdf || !weight) return fail(AZU_ERR_ARG, "null argument");
     const size_t n = (size_t)v->params.resolution * v->params.resolution * v->params.resolution;
@@ -450,6 +481,11 @@ static int volTransfer(azu_volume* v, float* tsdf, float* weight, bool download)
         }
         if (e != hipSuccess) rc = hipFail(e, download ? "volume download" : "volume upload");
     }
+    if (!download && rc == AZU_OK) {
+        hipError_t e = launchBrickMap(v->format, v->vox, v->params.resolution, v->bricks, true, nullptr);
+        if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+        if (e != hipSuccess) rc = hipFail(e, "brick map rebuild");
+    }
     (void)hipFree(dt);
     (void)hipFree(dw);
     return rc;
@@ -464,6 +500,7 @@ int azu_vol_storage(azu_volume* v, void** d_ptr, size_t* bytes) {
     if (!v) return fail(AZU_ERR_ARG, "null volume");
     if (d_ptr) *d_ptr = v->vox;
     if (bytes) *bytes = v->bytes;
+    v->external = true;
     return AZU_OK;
 }
 
@@ -515,7 +552,7 @@ static int integrateImpl(azu_volume* v, const float* d_depth, int W, int H, cons
     }
     AZU_HIP_TRY(launchIntegrate(v->format, exact, v->vox, toVol(v->params), Intr{K[0], K[1], K[2], K[3]}, W, H,
                                 d_depth, min_d, max_d, pa, v->dtr, v->recip, exact ? v->bounds : nullptr, v->counters,
-                                (hipStream_t)s),
+                                v->bricks, (hipStream_t)s),
                 "integrate launch");
     v->frames += 1;
     return AZU_OK;
@@ -542,8 +579,25 @@ static int raycastImpl(azu_volume* v, const PoseArg& pa, const float* K, int W, 
     ProfScope ps(exact ? (v->format == AZU_FMT_F32 ? "raycast_exact_f32" : "raycast_exact_f16")
                        : (v->format == AZU_FMT_F32 ? "raycast_fast_f32" : "raycast_fast_f16"),
                  (hipStream_t)s);
-    AZU_HIP_TRY(launchRaycast(v->format, exact, v->vox, toVol(v->params), v->dvs, pa, Intr{K[0], K[1], K[2], K[3]},
-                              W, H, dv, dn, dt, (hipStream_t)s),
+    const size_t px = (size_t)W * H;
+    if (px > v->dsafe_px) {
+        // grows only with the image; work queued on other streams may still use the old one
+        AZU_HIP_TRY(hipDeviceSynchronize(), "synchronise before skip scratch reallocation");
+        if (v->dsafe) (void)hipFree(v->dsafe);
+        v->dsafe = nullptr;
+        v->dsafe_px = 0;
+        if (hipMalloc(&v->dsafe, (px + 1) * sizeof(unsigned)) != hipSuccess)
+            return fail(AZU_ERR_ALLOC, "raycast skip scratch allocation failed");
+        AZU_HIP_TRY(launchSkipInit(v->dsafe, px, (hipStream_t)s), "skip scratch init");
+        v->dsafe_px = px;
+    }
+    if (v->skip && v->external)
+        AZU_HIP_TRY(launchBrickMap(v->format, v->vox, v->params.resolution, v->bricks, true, (hipStream_t)s),
+                    "brick map rebuild");
+    AZU_HIP_TRY(launchRaycast(v->format, exact, v->vox, toVol(v->params), v->dvs, v->skip ? v->bricks : nullptr,
+                              v->dsafe,
+                              reinterpret_cast<int*>(v->dsafe + v->dsafe_px), pa, Intr{K[0], K[1], K[2], K[3]}, W,
+                              H, dv, dn, dt, (hipStream_t)s),
                 "raycast launch");
     return AZU_OK;
 }

@@ -45,6 +45,40 @@ __device__ __forceinline__ float poseEl(const PoseArg& p, int i) { return p.dptr
 // ---------------------------------------------------------------------------
 enum Format : int { kFmtF32 = 0, kFmtF16U16 = 1 };
 
+// ---------------------------------------------------------------------------
+// Brick map (the raycast's empty-space skipping). The volume is cut into bricks of
+// kBrick^3 voxels; brick b (per axis) spans the voxels [kBrick b, kBrick b + kBrick],
+// one layer shared with the next brick, so every trilinear cell with its lower corner
+// in [kBrick b, kBrick b + kBrick - 1] has all eight corners inside one brick's span.
+// A brick's byte is 0 only while every voxel of its span is "far": unobserved, or
+// observed with tsdf in [kFarTsdf, 1.5]. A sample whose corners are all far is either
+// unobserved (invalid) or finite with value >= kFarTsdf (1 - 8 2^-24) > 0.999f (three
+// levels of linear interpolation, each losing at most 2.5 float roundings of a convex
+// combination of values >= kFarTsdf); either way the march takes its far step and
+// neither ends nor hits there. Writers mark bricks (byte = 1) for every voxel they leave
+// non-far, so the map only ever over-approximates the non-far voxels.
+// ---------------------------------------------------------------------------
+constexpr int kBrick = 16;
+constexpr float kFarTsdf = 0.999005f;  // <= half(0.99902344), > 0.999f (1 + 5e-6)
+
+__host__ __device__ inline int brickCount(int res) { return (res + kBrick - 1) / kBrick; }
+
+// Marks, in z-brick bz, every brick whose x and y spans hold voxel column (x, y): brick
+// v / kBrick on each axis, and the one below it when v is that brick's shared first layer.
+__device__ __forceinline__ void markBricksXY(unsigned char* __restrict__ bricks, int nb, int x, int y, int bz) {
+    const int bx = x / kBrick, by = y / kBrick;
+    const int nx = (x % kBrick == 0 && bx > 0) ? 2 : 1, ny = (y % kBrick == 0 && by > 0) ? 2 : 1;
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) bricks[((size_t)bz * nb + (by - j)) * nb + (bx - i)] = 1;
+}
+
+// Marks every brick whose span holds voxel (x, y, z).
+__device__ __forceinline__ void markBricks(unsigned char* __restrict__ bricks, int nb, int x, int y, int z) {
+    const int bz = z / kBrick;
+    markBricksXY(bricks, nb, x, y, bz);
+    if (z % kBrick == 0 && bz > 0) markBricksXY(bricks, nb, x, y, bz - 1);
+}
+
 struct StoreF32 {
     using Word = float2;
     static constexpr int kBytes = 8;
@@ -55,6 +89,10 @@ struct StoreF32 {
                                                      Word h) {
         return (a.y > 0.0f) & (b.y > 0.0f) & (c.y > 0.0f) & (d.y > 0.0f) & (e.y > 0.0f) & (f.y > 0.0f) &
                (g.y > 0.0f) & (h.y > 0.0f);
+    }
+    // far for the brick map: unobserved (as in observed8), or tsdf in [kFarTsdf, 1.5]
+    __device__ __forceinline__ static bool far(Word w) {
+        return !(w.y > 0.0f) || (w.x >= kFarTsdf && w.x <= 1.5f);
     }
     __device__ __forceinline__ static Word pack(float t, float wt) { return make_float2(t, wt); }
     __host__ __device__ static Word empty() { return float2{1.0f, 0.0f}; }
@@ -71,6 +109,10 @@ struct StoreF16 {
     __device__ __forceinline__ static bool observed8(Word a, Word b, Word c, Word d, Word e, Word f, Word g,
                                                      Word h) {
         return min(min(min(a, b), min(c, d)), min(min(e, f), min(g, h))) >= 0x10000u;
+    }
+    __device__ __forceinline__ static bool far(Word w) {
+        const float t = tsdf(w);
+        return w < 0x10000u || (t >= kFarTsdf && t <= 1.5f);
     }
     __device__ __forceinline__ static Word pack(float t, float wt) {
         const uint32_t h = __half_as_ushort(__float2half_rn(t));

@@ -13,6 +13,8 @@
 // Exact = true (FP32 parity path): IEEE division where the reference divides, no
 // contraction, bit-identical to the CPU restatement. Exact = false (FP16 form):
 // multiplications by 1/vs and fmaf interpolation.
+#include <algorithm>
+
 #include "common.hpp"
 
 #ifndef RAY_SPEC32
@@ -52,6 +54,17 @@ template <class S> struct Vol {
     float vs, inv_vs;
     float ox, oy, oz;
     FixedDiv dvs;  // division by vs
+};
+
+// Empty-space skipping: per pixel, a lower bound on the distance from the camera to
+// every marked brick whose span box the pixel's ray can meet (brickSplat), and a flag
+// saying whether the bounds hold for this call. With the bound, every sample at
+// t < dist / |dir| lies in no marked brick: each of its corners lies in a clear brick's
+// span (or the cell is outside the volume), so the sample is far, and the march takes
+// its far step there without reading it.
+struct RaySkip {
+    unsigned* dsafe;  // W*H float bit patterns (+inf between calls), or null
+    const int* ok;
 };
 
 // sampleTSDF (TSDFVolume.cpp:492-525) in two halves, so a ray can issue the reads of
@@ -178,7 +191,7 @@ template <class S> constexpr int kSpecOf = S::kBytes == 8 ? RAY_SPEC32 : RAY_SPE
 
 template <class S, bool E, int FD>
 __global__ __launch_bounds__(256) void raycastKernel(Vol<S> V, VolParams p, PoseArg pa, Intr K, int W, int H,
-                                                     RayOut out) {
+                                                     RayOut out, RaySkip skip) {
     // XCD-aware tile order: XCD k (blocks b with b % 8 == k) gets a contiguous band.
     const int ntx = (W + 15) / 16, nty = (H + 15) / 16, nt = ntx * nty;
     const int b = blockIdx.x, xcd = b & 7, k = b >> 3;
@@ -224,13 +237,41 @@ __global__ __launch_bounds__(256) void raycastKernel(Vol<S> V, VolParams p, Pose
         t_far = smin(t_far, t1);
     }
 
+    // A step of min_step must move t everywhere on [t_near, t_far]; where it cannot (a
+    // camera so far away that the float spacing of t exceeds twice the step) the march
+    // would never end, and the ray is left a miss.
+    if (!(fabsf(t_far) + min_step > fabsf(t_far)) || !(fabsf(t_near) + min_step > fabsf(t_near))) t_far = -1.0f;
+    // The skip bound of this pixel; the slot is left at +inf for the next call.
+    float t_safe = -__builtin_inff();
+    if (skip.dsafe) {
+        const float d = __uint_as_float(skip.dsafe[o]);
+        skip.dsafe[o] = 0x7f800000u;
+        if (*skip.ok) t_safe = d / f3norm(dir);
+    }
     bool found = false;
     float t_hit = 0.0f;
     F3 hit{0, 0, 0}, nrm{0, 0, 0};
     if (t_near < t_far) {
         float t_prev = t_near, f_prev = 1.0f;
-        bool prev_valid = sampleVol<S, E, FD>(V, f3add(cam, f3mul(dir, t_prev)), f_prev);
-        bool go = !(prev_valid && (!isfinite(f_prev) || f_prev < 0.0f));
+        bool prev_valid = false, go = true;
+        if (t_near < t_safe) {
+            // Samples before t_safe are far: advance by the far-step recurrence without
+            // taking them, then take the last one skipped (its value enters the next
+            // step's crossing test).
+            for (;;) {
+                const float t = smin(t_prev + far_step, t_far);
+                if (!(t < t_safe)) break;
+                t_prev = t;
+                if (t_prev >= t_far) {
+                    go = false;
+                    break;
+                }
+            }
+        }
+        if (go) {
+            prev_valid = sampleVol<S, E, FD>(V, f3add(cam, f3mul(dir, t_prev)), f_prev);
+            go = !(prev_valid && (!isfinite(f_prev) || f_prev < 0.0f));
+        }
         while (go && t_prev < t_far) {
             if (kSpecOf<S> > 0 && (!prev_valid || f_prev >= 0.999f)) {
                 // A run of far steps: while every sample is unobserved or >= 0.999 the
@@ -340,35 +381,153 @@ Vol<S> makeVol(const void* vox, const VolParams& p, const FixedDiv& dvs) {
 }
 
 template <class S, bool E, int FD>
-void launchRay(const void* vox, const VolParams& p, const FixedDiv& dvs, const PoseArg& pa, Intr K, int W, int H,
-               const RayOut& out, hipStream_t st) {
+void launchRay(const void* vox, const VolParams& p, const FixedDiv& dvs, const RaySkip& skip, const PoseArg& pa,
+               Intr K, int W, int H, const RayOut& out, hipStream_t st) {
     const int nt = ((W + 15) / 16) * ((H + 15) / 16);
     const unsigned blocks = (unsigned)(((nt + 7) / 8) * 8);
     hipLaunchKernelGGL((raycastKernel<S, E, FD>), dim3(blocks), dim3(256), 0, st, makeVol<S>(vox, p, dvs), p, pa, K,
-                       W, H, out);
+                       W, H, out, skip);
 }
 
 // dvs: division by the voxel size, verified exact (dvs.fast) or IEEE. The divided
 // values are sample coordinates minus the volume origin; when no origin component is
 // below 2^-60 in magnitude they cannot land in (0, 2^-100) and the unguarded form runs.
+// The per-pixel bounds of RaySkip. One wave per brick: a marked brick's span box,
+// widened by e voxels (e covers the rounding of the march's grid coordinates, which stays
+// below 4 2^-24 M voxels for a bound M on the magnitudes involved), is projected through
+// the inverse camera rotation. Only its part in front of the plane z = min_depth / 2 can
+// hold a sample (every sample lies at camera depth >= min_depth), so the corners there
+// and the box edges' crossings of the plane are projected; the perspective image of that
+// convex part is their hull, and the pixel rectangle around it, widened by one pixel,
+// takes the minimum of the box's distance from the camera (less 1e-5 of it), one pixel
+// per lane. A sample at parameter t is the point cam + dir t, at distance t |dir|.
+// ok = 0 (take every sample) unless the pose is finite and invertible and min_depth > 0.
+__global__ __launch_bounds__(256) void brickSplat(const unsigned char* __restrict__ bricks, int nb, VolParams p,
+                                                  PoseArg pa, Intr K, int W, int H, unsigned* __restrict__ dsafe,
+                                                  int* __restrict__ ok) {
+    const Rigid P = rigidFrom(pa);
+    const M33& R = P.R;
+    M33 A;  // adjugate of R
+    A.m[0][0] = R.m[1][1] * R.m[2][2] - R.m[1][2] * R.m[2][1];
+    A.m[0][1] = R.m[0][2] * R.m[2][1] - R.m[0][1] * R.m[2][2];
+    A.m[0][2] = R.m[0][1] * R.m[1][2] - R.m[0][2] * R.m[1][1];
+    A.m[1][0] = R.m[1][2] * R.m[2][0] - R.m[1][0] * R.m[2][2];
+    A.m[1][1] = R.m[0][0] * R.m[2][2] - R.m[0][2] * R.m[2][0];
+    A.m[1][2] = R.m[0][2] * R.m[1][0] - R.m[0][0] * R.m[1][2];
+    A.m[2][0] = R.m[1][0] * R.m[2][1] - R.m[1][1] * R.m[2][0];
+    A.m[2][1] = R.m[0][1] * R.m[2][0] - R.m[0][0] * R.m[2][1];
+    A.m[2][2] = R.m[0][0] * R.m[1][1] - R.m[0][1] * R.m[1][0];
+    const float det = R.m[0][0] * A.m[0][0] + R.m[0][1] * A.m[1][0] + R.m[0][2] * A.m[2][0];
+    bool valid = isfinite(det) && fabsf(det) > 1e-3f && p.min_depth > 0.0f && f3finite(P.t);
+    M33 Ri;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            Ri.m[r][c] = A.m[r][c] / det;
+            valid = valid && isfinite(Ri.m[r][c]);
+        }
+    if (blockIdx.x == 0 && threadIdx.x == 0) *ok = valid ? 1 : 0;
+    if (!valid) return;
+
+    const int lane = threadIdx.x & 63;
+    const size_t bi = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (bi >= (size_t)nb * nb * nb || bricks[bi] == 0) return;  // whole wave
+    const int b[3] = {(int)(bi % nb), (int)((bi / nb) % nb), (int)(bi / ((size_t)nb * nb))};
+    const F3 cam = P.t;
+    const float cam_a[3] = {cam.x, cam.y, cam.z};
+    const float M = (fabsf(cam.x) + fabsf(cam.y) + fabsf(cam.z) +
+                     2.0f * (fabsf(p.origin[0]) + fabsf(p.origin[1]) + fabsf(p.origin[2]))) / p.vs +
+                    12.0f * (float)p.res + 32.0f;
+    const float e = 0.01f + M * 0x1p-17f;
+    float lo[3], hi[3], d2 = 0.0f;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = p.origin[a] + p.vs * ((float)(kBrick * b[a]) - e);
+        hi[a] = p.origin[a] + p.vs * ((float)(kBrick * b[a] + kBrick) + e);
+        const float d = fmaxf(fmaxf(lo[a] - cam_a[a], cam_a[a] - hi[a]), 0.0f);
+        d2 += d * d;
+    }
+    const unsigned dlow = __float_as_uint(sqrtf(d2) * (1.0f - 1e-5f));
+    F3 q[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+        q[k] = mulv(Ri, F3{((k & 1) ? hi[0] : lo[0]) - cam.x, ((k & 2) ? hi[1] : lo[1]) - cam.y,
+                           ((k & 4) ? hi[2] : lo[2]) - cam.z});
+    const float zc = 0.5f * p.min_depth;
+    float umin = __builtin_inff(), umax = -__builtin_inff(), vmin = __builtin_inff(), vmax = -__builtin_inff();
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        if (q[k].z >= zc) {
+            const float u = K.fx * q[k].x / q[k].z + K.cx, v = K.fy * q[k].y / q[k].z + K.cy;
+            umin = fminf(umin, u);
+            umax = fmaxf(umax, u);
+            vmin = fminf(vmin, v);
+            vmax = fmaxf(vmax, v);
+        }
+#pragma unroll
+        for (int bit = 1; bit < 8; bit <<= 1) {
+            if (k & bit) continue;
+            const F3 qa = q[k], qb = q[k | bit];
+            if ((qa.z >= zc) == (qb.z >= zc)) continue;
+            const float s = (zc - qa.z) / (qb.z - qa.z);
+            const float x = qa.x + (qb.x - qa.x) * s, y = qa.y + (qb.y - qa.y) * s;
+            const float u = K.fx * x / zc + K.cx, v = K.fy * y / zc + K.cy;
+            umin = fminf(umin, u);
+            umax = fmaxf(umax, u);
+            vmin = fminf(vmin, v);
+            vmax = fmaxf(vmax, v);
+        }
+    }
+    if (!(umin <= umax)) return;  // nothing in front of the plane
+    const int x0 = (int)fmaxf(floorf(fmaxf(umin, -4.0f)) - 1.0f, 0.0f);
+    const int x1 = (int)fminf(ceilf(fminf(umax, (float)W + 4.0f)) + 1.0f, (float)(W - 1));
+    const int y0 = (int)fmaxf(floorf(fmaxf(vmin, -4.0f)) - 1.0f, 0.0f);
+    const int y1 = (int)fminf(ceilf(fminf(vmax, (float)H + 4.0f)) + 1.0f, (float)(H - 1));
+    if (x0 > x1 || y0 > y1) return;
+    const int w = x1 - x0 + 1, area = w * (y1 - y0 + 1);
+    for (int k = lane; k < area; k += 64) atomicMin(&dsafe[(size_t)(y0 + k / w) * W + (x0 + k % w)], dlow);
+}
+
+__global__ void fillWords(unsigned* __restrict__ p, size_t n, unsigned v) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// The skip bounds' scratch in its resting state (+inf everywhere).
+hipError_t launchSkipInit(unsigned* dsafe, size_t n, hipStream_t st) {
+    hipLaunchKernelGGL(fillWords, dim3((unsigned)std::min<size_t>((n + 255) / 256, 4096)), dim3(256), 0, st, dsafe,
+                       n, 0x7f800000u);
+    return hipGetLastError();
+}
+
+// bricks: the volume's brick map, or null to take every sample; dsafe (W*H words, all
+// +inf on entry and on return) and ok (one int): the skip bounds' scratch.
 hipError_t launchRaycast(int format, bool exact, const void* vox, const VolParams& p, const FixedDiv& dvs,
-                         const PoseArg& pa, Intr K, int W, int H, float* verts, float* norms, float* t_hit,
-                         hipStream_t st) {
+                         const unsigned char* bricks, unsigned* dsafe, int* ok, const PoseArg& pa, Intr K, int W,
+                         int H, float* verts, float* norms, float* t_hit, hipStream_t st) {
     const RayOut out{verts, norms, t_hit};
+    RaySkip skip{nullptr, ok};
+    if (bricks && dsafe && ok) {
+        const int nb = brickCount(p.res);
+        const unsigned blocks = (unsigned)(((size_t)nb * nb * nb + 3) / 4);
+        hipLaunchKernelGGL(brickSplat, dim3(blocks), dim3(256), 0, st, bricks, nb, p, pa, K, W, H, dsafe, ok);
+        skip.dsafe = dsafe;
+    }
     const bool far_origin = fabsf(p.origin[0]) >= 0x1p-60f && fabsf(p.origin[1]) >= 0x1p-60f &&
                             fabsf(p.origin[2]) >= 0x1p-60f;
     const int fd = dvs.fast == 0 ? 0 : (far_origin ? 2 : 1);
     if (format == kFmtF32) {
-        if (!exact) launchRay<StoreF32, false, 0>(vox, p, dvs, pa, K, W, H, out, st);
-        else if (fd == 2) launchRay<StoreF32, true, 2>(vox, p, dvs, pa, K, W, H, out, st);
-        else if (fd == 1) launchRay<StoreF32, true, 1>(vox, p, dvs, pa, K, W, H, out, st);
-        else launchRay<StoreF32, true, 0>(vox, p, dvs, pa, K, W, H, out, st);
+        if (!exact) launchRay<StoreF32, false, 0>(vox, p, dvs, skip, pa, K, W, H, out, st);
+        else if (fd == 2) launchRay<StoreF32, true, 2>(vox, p, dvs, skip, pa, K, W, H, out, st);
+        else if (fd == 1) launchRay<StoreF32, true, 1>(vox, p, dvs, skip, pa, K, W, H, out, st);
+        else launchRay<StoreF32, true, 0>(vox, p, dvs, skip, pa, K, W, H, out, st);
     } else {
         // The FP16 march is instruction-bound: the guarded form (a branch per division)
         // measures slower than the IEEE division there, so it takes only the unguarded one.
-        if (!exact) launchRay<StoreF16, false, 0>(vox, p, dvs, pa, K, W, H, out, st);
-        else if (fd == 2) launchRay<StoreF16, true, 2>(vox, p, dvs, pa, K, W, H, out, st);
-        else launchRay<StoreF16, true, 0>(vox, p, dvs, pa, K, W, H, out, st);
+        if (!exact) launchRay<StoreF16, false, 0>(vox, p, dvs, skip, pa, K, W, H, out, st);
+        else if (fd == 2) launchRay<StoreF16, true, 2>(vox, p, dvs, skip, pa, K, W, H, out, st);
+        else launchRay<StoreF16, true, 0>(vox, p, dvs, skip, pa, K, W, H, out, st);
     }
     return hipGetLastError();
 }

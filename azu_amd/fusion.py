@@ -67,6 +67,10 @@ class TSDFVolume:
     def set_mode(self, mode):
         check(lib.azu_vol_set_mode(self.h, mode))
 
+    def set_skip(self, on):
+        """Empty-space skipping in the raycast (on by default; same results either way)."""
+        check(lib.azu_vol_set_skip(self.h, 1 if on else 0))
+
     def setParams(self, params):
         cp = params.c()
         check(lib.azu_vol_set_params(self.h, ctypes.byref(cp)))

@@ -87,6 +87,7 @@ _SIGS = {
     "azu_vol_set_params": (ctypes.c_int, [c_void_p, ctypes.POINTER(TsdfParams)]),
     "azu_vol_get_params": (ctypes.c_int, [c_void_p, ctypes.POINTER(TsdfParams), ctypes.POINTER(ctypes.c_int)]),
     "azu_vol_set_mode": (ctypes.c_int, [c_void_p, ctypes.c_int]),
+    "azu_vol_set_skip": (ctypes.c_int, [c_void_p, ctypes.c_int]),
     "azu_vol_division_check": (ctypes.c_int, [c_void_p] + [ctypes.POINTER(ctypes.c_int)] * 3),
     "azu_vol_download": (ctypes.c_int, [c_void_p, c_void_p, c_void_p]),
     "azu_vol_upload": (ctypes.c_int, [c_void_p, c_void_p, c_void_p]),

@@ -121,6 +121,11 @@ int azu_vol_reset(azu_volume* v, azu_stream s);
 int azu_vol_set_params(azu_volume* v, const azu_tsdf_params* p);
 int azu_vol_get_params(const azu_volume* v, azu_tsdf_params* p, int* format);
 int azu_vol_set_mode(azu_volume* v, int mode);
+/* Empty-space skipping in the raycast (on by default; results are the same either way).
+ * Off, every sample is taken and no brick map is read or rebuilt: for callers that write
+ * voxels through azu_vol_storage and do not want a map rebuild per raycast. No
+ * counterpart in the reference. */
+int azu_vol_set_skip(azu_volume* v, int on);
 /* Diagnostic: whether the EXACT kernels divide by the voxel size / the truncation with
  * a reciprocal and two exact corrections (1) or with an IEEE division (0), and take the
  * camera depth's reciprocal by Newton-Raphson (1) or IEEE (0). Each fast form is chosen
@@ -133,7 +138,10 @@ int azu_vol_division_check(const azu_volume* v, int* voxel_size_fast, int* trunc
  * Synchronise. */
 int azu_vol_download(azu_volume* v, float* tsdf, float* weight);
 int azu_vol_upload(azu_volume* v, const float* tsdf, const float* weight);
-/* Raw device storage (format-dependent words, index as above). */
+/* Raw device storage (format-dependent words, index as above). The raycast skips empty
+ * space with a brick map that integrate / reset / upload keep; once the storage has been
+ * handed out, writes through it are possible and every later raycast of this volume
+ * first rebuilds the map from the voxels (a full read of the volume). */
 int azu_vol_storage(azu_volume* v, void** d_ptr, size_t* bytes);
 /* Voxels updated / newly observed by integrate since the last reset of the counters
  * (reset != 0 clears them after reading). Synchronises. */
@@ -150,7 +158,10 @@ int azu_integrate_dpose(azu_volume* v, const float* d_depth_m, int W, int H, con
 
 /* TSDFVolume::raycast / raycastGPU (TSDFVolume.cpp:348-476, TSDFVolume.h:99-104,
  * 177-181): W*H packed xyz vertices and normals on the device; a miss is all zeros.
- * d_t_hit (optional, may be NULL) receives the ray parameter of each hit (NaN on a miss). */
+ * d_t_hit (optional, may be NULL) receives the ray parameter of each hit (NaN on a miss).
+ * Samples that the volume's brick map proves to be far (unobserved, or tsdf >= 0.999)
+ * are not read; results are the same as taking them. Raycasts of one volume share its
+ * skip scratch: issue them on one stream (or synchronise between them). */
 int azu_raycast(azu_volume* v, const float pose[16], const float K[4], int W, int H, float* d_vertices,
                 float* d_normals, azu_stream s);
 int azu_raycast_dpose(azu_volume* v, const float* d_pose, const float K[4], int W, int H, float* d_vertices,

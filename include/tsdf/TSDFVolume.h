@@ -173,6 +173,10 @@ public:
 
     azu_volume* handle() const { return h_; }
     void setStream(azu_stream s) { stream_ = s; }
+    // Empty-space skipping in the raycast (on by default; the same results either way).
+    void setEmptySpaceSkipping(bool on) {
+        detail::check(azu_vol_set_skip(h_, on ? 1 : 0), "TSDFVolume::setEmptySpaceSkipping");
+    }
     void setGPUEnabled(bool) {}  // always on the GPU
     bool isGPUEnabled() const { return true; }
 
