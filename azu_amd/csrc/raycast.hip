@@ -108,9 +108,18 @@ __device__ __forceinline__ Cell<S> sampleLoad(const Vol<S>& V, F3 p) {
 }
 
 template <class S, bool E>
+__device__ __forceinline__ bool sampleEvalValue(const Cell<S>& c, float& f);
+
+template <class S, bool E>
 __device__ __forceinline__ bool sampleEval(const Cell<S>& c, float& f) {
     if (!c.inside) return false;
     if (!S::observed8(c.q0.a, c.q0.b, c.q1.a, c.q1.b, c.q2.a, c.q2.b, c.q3.a, c.q3.b)) return false;
+    return sampleEvalValue<S, E>(c, f);
+}
+
+// The trilinear value of a cell (no gates).
+template <class S, bool E>
+__device__ __forceinline__ bool sampleEvalValue(const Cell<S>& c, float& f) {
     const float tx = c.tx, ty = c.ty, tz = c.tz;
     const float c0 = S::tsdf(c.q0.a), c1 = S::tsdf(c.q0.b), c2 = S::tsdf(c.q1.a), c3 = S::tsdf(c.q1.b);
     const float c4 = S::tsdf(c.q2.a), c5 = S::tsdf(c.q2.b), c6 = S::tsdf(c.q3.a), c7 = S::tsdf(c.q3.b);
@@ -129,6 +138,20 @@ __device__ __forceinline__ bool sampleEval(const Cell<S>& c, float& f) {
         f = fmaf(v1 - v0, tz, v0);
     }
     return true;
+}
+
+// sampleEval without branches: the value is computed from whatever was read and
+// replaced by 1 where the sample is invalid (outside, or a corner unobserved).
+template <class S, bool E>
+__device__ __forceinline__ bool sampleEvalFlat(const Cell<S>& c, float& f) {
+    const bool valid =
+        c.inside && S::observed8(c.q0.a, c.q0.b, c.q1.a, c.q1.b, c.q2.a, c.q2.b, c.q3.a, c.q3.b);
+    Cell<S> d = c;
+    d.inside = true;
+    float v = 1.0f;
+    (void)sampleEvalValue<S, E>(d, v);
+    f = valid ? v : 1.0f;
+    return valid;
 }
 
 template <class S, bool E, int FD>
@@ -328,23 +351,23 @@ __global__ __launch_bounds__(256) void raycastKernel(Vol<S> V, VolParams p, Pose
             const float step =
                 (!prev_valid || f_prev >= 0.999f) ? far_step : smax(min_step, 0.8f * f_prev * trunc);
             const float t = smin(t_prev + step, t_far);
-            float f = 1.0f;
-            const bool valid = sampleVol<S, E, FD>(V, f3add(cam, f3mul(dir, t)), f);
-            if (valid && !isfinite(f)) break;
-            if (valid && prev_valid) {
-                if (f_prev > 0.0f && f <= 0.0f) {
-                    t_hit = t_prev + (t - t_prev) * (f_prev / (f_prev - f));
-                    found = true;
-                    break;
-                }
-                if (f_prev < 0.0f && f >= 0.0f) break;
-            } else if (valid && f < 0.0f) {
-                break;
+            // One step with its outcomes as flags (one exit branch): a non-finite value
+            // stops the march; with both samples valid, + -> - is the hit (secant step)
+            // and - -> + stops; a valid negative first sample after an invalid one stops.
+            float f;
+            const bool valid = sampleEvalFlat<S, E>(sampleLoad<S, E, FD>(V, f3add(cam, f3mul(dir, t))), f);
+            const bool fin = isfinite(f), both = valid && prev_valid;
+            const bool hit = both && fin && f_prev > 0.0f && f <= 0.0f;
+            const bool stop = hit || (valid && !fin) || (both && f_prev < 0.0f && f >= 0.0f) ||
+                              (valid && !prev_valid && f < 0.0f) || t >= t_far;
+            if (hit) {
+                t_hit = t_prev + (t - t_prev) * (f_prev / (f_prev - f));
+                found = true;
             }
             t_prev = t;
             f_prev = f;
             prev_valid = valid;
-            if (t >= t_far) break;
+            if (stop) break;
         }
         if (found) {
             hit = f3add(cam, f3mul(dir, t_hit));
