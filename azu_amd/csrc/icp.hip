@@ -27,6 +27,16 @@ constexpr int kIcpSlots = kIcpSlotsF + kIcpSlotsI;
 constexpr int kIcpMaxBlocks = 256;
 constexpr int kSumGroups = 7;  // last block: 7 threads per slot (7 * 34 = 238 <= 256)
 
+// Pose terms every thread of an iteration uses: the current pose (camera to world) and
+// the transform from the live camera into the reference camera. Computed once per
+// pose change (icpInit, the last block's update) into the state, which every block of
+// the next launch reads at its start.
+struct PoseDerived {
+    M33 R_cw, R_rel;
+    F3 t_cw, t_rel;
+    int iterations;  // the state's packed step counts for the next launch
+};
+
 struct IcpState {
     float pose[16];
     float ref[16];
@@ -37,12 +47,13 @@ struct IcpState {
     int inliers;
     int converged;
     int counters[5];  // valid_live, valid_model, projected, dist_filtered, angle_filtered
-    int stop_level;   // level whose iteration loop has ended, -1 while none
     int iterations;   // accepted steps, one byte per level (level l at bit 8*l)
     // Phase timing of the last block, in 100 MHz clock ticks summed over the launches
     // that reached it: cross-block sum, solve; and the number of such launches. Work of
     // the same launches: interior live pixels visited, in-bounds model projections.
     unsigned long long ticks_sum, ticks_solve, ticks_n, work_px, work_proj;
+    PoseDerived next;  // the pose terms of pose / ref / iterations above
+    int stop_level;    // level whose iteration loop has ended, -1 while none
 };
 
 struct IcpIterArgs {
@@ -305,12 +316,18 @@ __device__ void projectSO3(const double* M, double* R) {
     for (int i = 0; i < 9; ++i) R[i] = X[i];
 }
 
-// Pose terms every thread of a launch uses, computed once per block into LDS.
-struct PoseDerived {
-    M33 R_cw, R_rel;
-    F3 t_cw, t_rel;
-    int iterations;  // the state's packed step counts when the launch began
-};
+
+// The pose terms of a pose and reference pose (column-major 4 x 4 each).
+__device__ PoseDerived derivePose(const float* pose, const float* ref, int iterations) {
+    const Rigid P = rigidFromPtr(pose), rinv = rigidInverse(rigidFromPtr(ref));
+    PoseDerived d;
+    d.R_cw = P.R;
+    d.t_cw = P.t;
+    d.R_rel = mulm(rinv.R, P.R);
+    d.t_rel = f3add(mulv(rinv.R, P.t), rinv.t);
+    d.iterations = iterations;
+    return d;
+}
 
 // One solve + pose update (trackLevel's loop body, ICPTracker.cpp:79-130), lane 0 of
 // the last block once solveLane has run on lanes 0-2. sums: 28 FP64 slots (21 A upper,
@@ -406,13 +423,19 @@ __device__ void icpSolve(IcpState* st, const double* sums, const SolveLanes* sl,
     }
     double Rp[9];
     projectSO3(Md, Rp);
+    float np[16];
+    for (int i = 0; i < 16; ++i) np[i] = st->pose[i];  // the bottom row stays as it was
     for (int i = 0; i < 3; ++i) {
-        for (int j = 0; j < 3; ++j) st->pose[j * 4 + i] = (float)Rp[i * 3 + j];
-        st->pose[12 + i] = nt[i];
+        for (int j = 0; j < 3; ++j) np[j * 4 + i] = (float)Rp[i * 3 + j];
+        np[12 + i] = nt[i];
     }
+    for (int i = 0; i < 16; ++i) st->pose[i] = np[i];
     st->final_step = step_norm;
     st->last_final_step = step_norm;
     st->iterations = pd.iterations + (1 << (8 * level));
+    float rf[16];
+    for (int i = 0; i < 16; ++i) rf[i] = st->ref[i];
+    st->next = derivePose(np, rf, pd.iterations + (1 << (8 * level)));
     st->error = (float)(residual / (double)(inliers > 1 ? inliers : 1));
     st->inliers = inliers;
     if (step_norm < 5e-5f) {
@@ -442,6 +465,7 @@ __global__ void icpInit(IcpState* st, PoseArg est, PoseArg ref) {
     for (int i = 0; i < 5; ++i) st->counters[i] = 0;
     st->stop_level = -1;
     st->iterations = 0;
+    st->next = derivePose(e, r, 0);
 }
 
 __global__ void icpClearTicks(IcpState* st) {
@@ -593,7 +617,6 @@ __host__ __device__ int icpBlocksFor(int W, int H) {
 }
 
 struct IcpShared {
-    PoseDerived sp;
     float red[2 * kIcpSlotsF][256];  // every thread's compensated pairs, slot-major
     double part[kIcpSlotsF][kFoldGroups];
     int cnt[4][kIcpSlotsI];
@@ -603,13 +626,6 @@ struct IcpShared {
     int last;
 };
 
-__device__ __forceinline__ void derivePose(IcpShared& sh, const Rigid& pose, const Rigid& rinv, int iterations) {
-    sh.sp.R_cw = pose.R;
-    sh.sp.t_cw = pose.t;
-    sh.sp.R_rel = mulm(rinv.R, pose.R);
-    sh.sp.t_rel = f3add(mulv(rinv.R, pose.t), rinv.t);
-    sh.sp.iterations = iterations;
-}
 
 // One block's share of an iteration: the pixel pass over its pixels, the block fold and
 // its FP64 partials (written by wave 0). The level is split over nblk blocks (blocks
@@ -745,14 +761,12 @@ __device__ void icpLastBlock(IcpState* __restrict__ st, const double* __restrict
 __global__ __launch_bounds__(256) void icpIter(IcpIterArgs a, IcpState* __restrict__ st,
                                                double* __restrict__ partials, unsigned* __restrict__ ticket) {
     __shared__ IcpShared sh;
-    if (st->stop_level == a.level) return;  // uniform over the grid
+    // The stop flag and the pose terms, read together (uniform over the grid; written
+    // only by a last block, after every block of its launch has read them).
+    const int stop = st->stop_level;
+    const PoseDerived pd = st->next;
+    if (stop == a.level) return;
     const int tid = threadIdx.x;
-    if (tid == 0) {
-        const Rigid ref = rigidFromPtr(st->ref);
-        derivePose(sh, rigidFromPtr(st->pose), rigidInverse(ref), st->iterations);
-    }
-    __syncthreads();
-    const PoseDerived pd = sh.sp;
     const int npx = icpBlockPass(a, pd, partials, sh, gridDim.x);
     __syncthreads();
     if (tid == 0) sh.last = arriveLast(ticket, blockIdx.x, gridDim.x) ? 1 : 0;
