@@ -235,6 +235,25 @@ __device__ __forceinline__ int tileClass(const IntFrame& f, const VolParams& p, 
 
 static_assert(kChunk == 2 * kBrick, "a tile spans two bricks in x");
 
+// One voxel word through a raw buffer resource (byte offsets: lane + uniform).
+template <class S> struct BufWord;
+template <> struct BufWord<StoreF16> {
+    __device__ static uint32_t load(__amdgpu_buffer_rsrc_t r, unsigned off, unsigned soff) {
+        return __builtin_amdgcn_raw_buffer_load_b32(r, off, soff, 0);
+    }
+    __device__ static void store(uint32_t v, __amdgpu_buffer_rsrc_t r, unsigned off, unsigned soff) {
+        __builtin_amdgcn_raw_buffer_store_b32(v, r, off, soff, 0);
+    }
+};
+template <> struct BufWord<StoreF32> {
+    __device__ static float2 load(__amdgpu_buffer_rsrc_t r, unsigned off, unsigned soff) {
+        return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, off, soff, 0));
+    }
+    __device__ static void store(float2 v, __amdgpu_buffer_rsrc_t r, unsigned off, unsigned soff) {
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0)), v), r, off, soff, 0);
+    }
+};
+
 // Brick-map bytes of one aligned tile half (integrateExact): x columns xc + j, j < 32,
 // lie in x-brick bx = xc / 16 + j / 16 (lane bits j and 32 + j); columns 0 and 16 are
 // also the first layers of bricks bx - 1 and bx. y = 16 q is also in y-brick q - 1.
@@ -279,7 +298,7 @@ __global__ __launch_bounds__(256) void integrateExact(typename S::Word* __restri
     const int ny = f.y1 - f.y0 + 1;
     const int zbase = f.z0 & ~63;
     const int nzg = (f.z1 - zbase + 1 + 63) / 64;
-    const int task = blockIdx.x * 4 + wv;
+    const int task = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wv);  // wave-uniform
     if (task >= ny * nzg) return;  // whole wave; the kernel uses no block barrier
     const int y = f.y0 + task % ny;
     const int zb = zbase + (task / ny) * 64;
@@ -344,7 +363,16 @@ __global__ __launch_bounds__(256) void integrateExact(typename S::Word* __restri
         // load is issued before the first fold.
         const int j = lane & (kChunk - 1), g = lane >> 5;
         const bool col_ok = j >= js && j < n;
-        typename S::Word* col = vox + ((size_t)y * res + (size_t)(xc + j));
+        // The tile's words through a buffer resource: base = the tile's first voxel
+        // (wave-uniform), lane offset (g plane + j) words, row offset 2 r plane words
+        // (uniform). The range is capped at 2^31 bytes (64 planes of at most 2048^2
+        // words stay below it), so the lane offset 2^31 reads 0 with no memory access.
+        const size_t tile0 = (size_t)zb * plane + (size_t)y * res + (size_t)xc;
+        const size_t left = (res * plane - tile0) * sizeof(typename S::Word);
+        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+            vox + tile0, (short)0, (int)(left < 0x80000000ull ? left : 0x80000000ull), 0x00020000);
+        const unsigned lane_off = (unsigned)(((size_t)g * plane + (size_t)j) * sizeof(typename S::Word));
+        const unsigned row_step = (unsigned)(2 * plane * sizeof(typename S::Word));
 #pragma unroll 1
         for (int k0 = 0; k0 < 32; k0 += 16) {
             float tn[16];
@@ -355,11 +383,12 @@ __global__ __launch_bounds__(256) void integrateExact(typename S::Word* __restri
                 const float t = cls == kTileFree ? 1.0f : L[r * kLdsStride + j];
                 tn[k] = (col_ok && zb + r >= f.z0 && zb + r <= f.z1) ? t : __builtin_nanf("");
             }
-            // Lanes with nothing to fold read voxel 0 instead (a cached line), so the
-            // loads carry no branch and issue back to back.
+            // Lanes with nothing to fold read out of range (no access), so the loads
+            // carry no branch and issue back to back.
 #pragma unroll
             for (int k = 0; k < 16; ++k)
-                wd[k] = *(__builtin_isnan(tn[k]) ? vox : col + (size_t)(zb + g + 2 * (k0 + k)) * plane);
+                wd[k] = BufWord<S>::load(rs, __builtin_isnan(tn[k]) ? 0x80000000u : lane_off,
+                                         (unsigned)(k0 + k) * row_step);
             // Brick map: with the tile aligned, rows k0 + k of either lane half lie in
             // z-brick zb / 16 + k0 / 8 for k < 8 and the next one for k >= 8; near_h are the
             // lanes that left a non-far voxel there, shared_h the lower-half lanes whose
@@ -376,7 +405,7 @@ __global__ __launch_bounds__(256) void integrateExact(typename S::Word* __restri
                     newly = (w_old == 0.0f);
                     const typename S::Word nw =
                         S::pack((t_old * w_old + tn[k]) / (w_old + 1.0f), smin(w_old + 1.0f, p.max_w));
-                    col[(size_t)(zb + g + 2 * (k0 + k)) * plane] = nw;
+                    BufWord<S>::store(nw, rs, lane_off, (unsigned)(k0 + k) * row_step);
                     nf = !S::far(nw);
                 }
                 n_upd += upd ? 1u : 0u;
