@@ -36,6 +36,9 @@ template <> struct PairIO<StoreF32> {
         __builtin_memcpy(&r, p, sizeof(P));
         return r;
     }
+    __device__ static P loadB(__amdgpu_buffer_rsrc_t r, unsigned off, unsigned soff) {
+        return __builtin_bit_cast(P, __builtin_amdgcn_raw_buffer_load_b128(r, off, soff, 0));
+    }
 };
 template <> struct PairIO<StoreF16> {
     struct P {
@@ -46,6 +49,9 @@ template <> struct PairIO<StoreF16> {
         __builtin_memcpy(&r, p, sizeof(P));
         return r;
     }
+    __device__ static P loadB(__amdgpu_buffer_rsrc_t r, unsigned off, unsigned soff) {
+        return __builtin_bit_cast(P, __builtin_amdgcn_raw_buffer_load_b64(r, off, soff, 0));
+    }
 };
 
 template <class S> struct Vol {
@@ -54,6 +60,15 @@ template <class S> struct Vol {
     float vs, inv_vs;
     float ox, oy, oz;
     FixedDiv dvs;  // division by vs
+};
+
+// FP32 volumes of at most 2^31 bytes are read through a buffer resource over the whole
+// volume (wave-uniform), a 32-bit byte offset per sample and uniform row / plane
+// offsets for the cell's four corner pairs; an offset of 2^31 (a cell outside the
+// volume) reads 0 with no memory access.
+struct VolBuf {
+    __amdgpu_buffer_rsrc_t rs;
+    unsigned sy, sz;  // bytes between rows / planes
 };
 
 // Empty-space skipping: per pixel, a lower bound on the distance from the camera to
@@ -77,8 +92,8 @@ template <class S> struct Cell {
     bool inside;
 };
 
-template <class S, bool E, int FD>
-__device__ __forceinline__ Cell<S> sampleLoad(const Vol<S>& V, F3 p) {
+template <class S, bool E, int FD, bool B>
+__device__ __forceinline__ Cell<S> sampleLoad(const Vol<S>& V, const VolBuf& R, F3 p) {
     Cell<S> c;
     float gx, gy, gz;
     if (E) {
@@ -97,9 +112,20 @@ __device__ __forceinline__ Cell<S> sampleLoad(const Vol<S>& V, F3 p) {
     c.tx = gx - fx0;
     c.ty = gy - fy0;
     c.tz = gz - fz0;
+    using IO = PairIO<S>;
+    if (B) {
+        const unsigned off =
+            c.inside ? (((unsigned)z0 * (unsigned)V.res + (unsigned)y0) * (unsigned)V.res + (unsigned)x0) *
+                           (unsigned)S::kBytes
+                     : 0x80000000u;
+        c.q0 = IO::loadB(R.rs, off, 0u);
+        c.q1 = IO::loadB(R.rs, off, R.sy);
+        c.q2 = IO::loadB(R.rs, off, R.sz);
+        c.q3 = IO::loadB(R.rs, off, R.sy + R.sz);
+        return c;
+    }
     const size_t sy = (size_t)V.res, sz = (size_t)V.res * (size_t)V.res;
     const size_t b = (size_t)z0 * sz + (size_t)y0 * sy + (size_t)x0;
-    using IO = PairIO<S>;
     c.q0 = IO::load(V.vox + b);
     c.q1 = IO::load(V.vox + b + sy);
     c.q2 = IO::load(V.vox + b + sz);
@@ -154,8 +180,9 @@ __device__ __forceinline__ bool sampleEvalFlat(const Cell<S>& c, float& f) {
     return valid;
 }
 
-template <class S, bool E, int FD>
-__device__ __forceinline__ bool sampleVol(const Vol<S>& V, F3 p, float& f) {
+template <class S, bool E, int FD, bool B>
+__device__ __forceinline__ bool sampleVol(const Vol<S>& V, const VolBuf& R, F3 p, float& f) {
+    if (B) return sampleEval<S, E>(sampleLoad<S, E, FD, true>(V, R, p), f);
     float gx, gy, gz;
     if (E) {
         gx = divBy<FD>(V.dvs, p.x - V.ox);
@@ -185,16 +212,16 @@ __device__ __forceinline__ bool sampleVol(const Vol<S>& V, F3 p, float& f) {
 }
 
 // computeNormal (TSDFVolume.cpp:478-490)
-template <class S, bool E, int FD>
-__device__ __forceinline__ bool normalVol(const Vol<S>& V, F3 p, F3& n) {
+template <class S, bool E, int FD, bool B>
+__device__ __forceinline__ bool normalVol(const Vol<S>& V, const VolBuf& R, F3 p, F3& n) {
     const float h = V.vs;
     float s0, s1, s2, s3, s4, s5;
-    if (!sampleVol<S, E, FD>(V, F3{p.x + h, p.y + 0.0f, p.z + 0.0f}, s0) ||
-        !sampleVol<S, E, FD>(V, F3{p.x - h, p.y - 0.0f, p.z - 0.0f}, s1) ||
-        !sampleVol<S, E, FD>(V, F3{p.x + 0.0f, p.y + h, p.z + 0.0f}, s2) ||
-        !sampleVol<S, E, FD>(V, F3{p.x - 0.0f, p.y - h, p.z - 0.0f}, s3) ||
-        !sampleVol<S, E, FD>(V, F3{p.x + 0.0f, p.y + 0.0f, p.z + h}, s4) ||
-        !sampleVol<S, E, FD>(V, F3{p.x - 0.0f, p.y - 0.0f, p.z - h}, s5))
+    if (!sampleVol<S, E, FD, B>(V, R, F3{p.x + h, p.y + 0.0f, p.z + 0.0f}, s0) ||
+        !sampleVol<S, E, FD, B>(V, R, F3{p.x - h, p.y - 0.0f, p.z - 0.0f}, s1) ||
+        !sampleVol<S, E, FD, B>(V, R, F3{p.x + 0.0f, p.y + h, p.z + 0.0f}, s2) ||
+        !sampleVol<S, E, FD, B>(V, R, F3{p.x - 0.0f, p.y - h, p.z - 0.0f}, s3) ||
+        !sampleVol<S, E, FD, B>(V, R, F3{p.x + 0.0f, p.y + 0.0f, p.z + h}, s4) ||
+        !sampleVol<S, E, FD, B>(V, R, F3{p.x - 0.0f, p.y - 0.0f, p.z - h}, s5))
         return false;
     const F3 g{s0 - s1, s2 - s3, s4 - s5};
     const float len = f3norm(g);
@@ -212,9 +239,18 @@ struct RayOut {
 // Far-step samples whose reads one ray issues together, per storage form.
 template <class S> constexpr int kSpecOf = S::kBytes == 8 ? RAY_SPEC32 : RAY_SPEC16;
 
-template <class S, bool E, int FD>
+template <class S, bool E, int FD, bool B>
 __global__ __launch_bounds__(256) void raycastKernel(Vol<S> V, VolParams p, PoseArg pa, Intr K, int W, int H,
                                                      RayOut out, RaySkip skip) {
+    VolBuf R;
+    if (B) {
+        R.rs = __builtin_amdgcn_make_buffer_rsrc(const_cast<typename S::Word*>(V.vox), (short)0,
+                                                 (int)((unsigned)V.res * (unsigned)V.res * (unsigned)V.res *
+                                                       (unsigned)S::kBytes),
+                                                 0x00020000);
+        R.sy = (unsigned)V.res * (unsigned)S::kBytes;
+        R.sz = R.sy * (unsigned)V.res;
+    }
     // XCD-aware tile order: XCD k (blocks b with b % 8 == k) gets a contiguous band.
     const int ntx = (W + 15) / 16, nty = (H + 15) / 16, nt = ntx * nty;
     const int b = blockIdx.x, xcd = b & 7, k = b >> 3;
@@ -292,7 +328,7 @@ __global__ __launch_bounds__(256) void raycastKernel(Vol<S> V, VolParams p, Pose
             }
         }
         if (go) {
-            prev_valid = sampleVol<S, E, FD>(V, f3add(cam, f3mul(dir, t_prev)), f_prev);
+            prev_valid = sampleVol<S, E, FD, B>(V, R, f3add(cam, f3mul(dir, t_prev)), f_prev);
             go = !(prev_valid && (!isfinite(f_prev) || f_prev < 0.0f));
         }
         while (go && t_prev < t_far) {
@@ -312,7 +348,7 @@ __global__ __launch_bounds__(256) void raycastKernel(Vol<S> V, VolParams p, Pose
                     ts[k] = tt;
                 }
 #pragma unroll
-                for (int k = 0; k < kSpec; ++k) cs[k] = sampleLoad<S, E, FD>(V, f3add(cam, f3mul(dir, ts[k])));
+                for (int k = 0; k < kSpec; ++k) cs[k] = sampleLoad<S, E, FD, B>(V, R, f3add(cam, f3mul(dir, ts[k])));
 #pragma unroll
                 for (int k = 0; k < kSpec; ++k) {
                     const float t = ts[k];
@@ -355,7 +391,7 @@ __global__ __launch_bounds__(256) void raycastKernel(Vol<S> V, VolParams p, Pose
             // stops the march; with both samples valid, + -> - is the hit (secant step)
             // and - -> + stops; a valid negative first sample after an invalid one stops.
             float f;
-            const bool valid = sampleEvalFlat<S, E>(sampleLoad<S, E, FD>(V, f3add(cam, f3mul(dir, t))), f);
+            const bool valid = sampleEvalFlat<S, E>(sampleLoad<S, E, FD, B>(V, R, f3add(cam, f3mul(dir, t))), f);
             const bool fin = isfinite(f), both = valid && prev_valid;
             const bool hit = both && fin && f_prev > 0.0f && f <= 0.0f;
             const bool stop = hit || (valid && !fin) || (both && f_prev < 0.0f && f >= 0.0f) ||
@@ -371,7 +407,7 @@ __global__ __launch_bounds__(256) void raycastKernel(Vol<S> V, VolParams p, Pose
         }
         if (found) {
             hit = f3add(cam, f3mul(dir, t_hit));
-            found = normalVol<S, E, FD>(V, hit, nrm) && f3finite(hit);
+            found = normalVol<S, E, FD, B>(V, R, hit, nrm) && f3finite(hit);
         }
     }
     if (!found) {
@@ -408,8 +444,15 @@ void launchRay(const void* vox, const VolParams& p, const FixedDiv& dvs, const R
                Intr K, int W, int H, const RayOut& out, hipStream_t st) {
     const int nt = ((W + 15) / 16) * ((H + 15) / 16);
     const unsigned blocks = (unsigned)(((nt + 7) / 8) * 8);
-    hipLaunchKernelGGL((raycastKernel<S, E, FD>), dim3(blocks), dim3(256), 0, st, makeVol<S>(vox, p, dvs), p, pa, K,
-                       W, H, out, skip);
+    // The buffer form measures faster for the FP32 words (344 against 412 us per launch
+    // at 512^3) and slightly slower for the FP16 ones (224 against 218 us).
+    const bool small = S::kBytes == 8 && (size_t)p.res * p.res * p.res * S::kBytes <= ((size_t)1 << 31);
+    if (small)
+        hipLaunchKernelGGL((raycastKernel<S, E, FD, true>), dim3(blocks), dim3(256), 0, st, makeVol<S>(vox, p, dvs),
+                           p, pa, K, W, H, out, skip);
+    else
+        hipLaunchKernelGGL((raycastKernel<S, E, FD, false>), dim3(blocks), dim3(256), 0, st,
+                           makeVol<S>(vox, p, dvs), p, pa, K, W, H, out, skip);
 }
 
 // dvs: division by the voxel size, verified exact (dvs.fast) or IEEE. The divided
