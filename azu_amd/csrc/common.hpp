@@ -94,6 +94,8 @@ struct StoreF32 {
     __device__ __forceinline__ static bool far(Word w) {
         return !(w.y > 0.0f) || (w.x >= kFarTsdf && w.x <= 1.5f);
     }
+    // far for a word packed from tsdf t with a weight >= 1 (what integrate writes)
+    __device__ __forceinline__ static bool farPacked(float t) { return t >= kFarTsdf && t <= 1.5f; }
     __device__ __forceinline__ static Word pack(float t, float wt) { return make_float2(t, wt); }
     __host__ __device__ static Word empty() { return float2{1.0f, 0.0f}; }
 };
@@ -114,6 +116,10 @@ struct StoreF16 {
         const float t = tsdf(w);
         return w < 0x10000u || (t >= kFarTsdf && t <= 1.5f);
     }
+    // far for a word packed from tsdf t with a weight >= 1: binary16 rounding (to
+    // nearest, ties to even) lands in [0.99902344, 1.5] exactly for t in
+    // [0.99877930, 1.50048828] (the midpoints, whose ties go to the even ends inside)
+    __device__ __forceinline__ static bool farPacked(float t) { return t >= 0.99877930f && t <= 1.50048828f; }
     __device__ __forceinline__ static Word pack(float t, float wt) {
         const uint32_t h = __half_as_ushort(__float2half_rn(t));
         return h | (static_cast<uint32_t>(wt) << 16);

@@ -373,6 +373,9 @@ __global__ __launch_bounds__(256) void integrateExact(typename S::Word* __restri
             vox + tile0, (short)0, (int)(left < 0x80000000ull ? left : 0x80000000ull), 0x00020000);
         const unsigned lane_off = (unsigned)(((size_t)g * plane + (size_t)j) * sizeof(typename S::Word));
         const unsigned row_step = (unsigned)(2 * plane * sizeof(typename S::Word));
+        // rows r with zb + g + 2 r in [z0, z1] (none for a column outside the box)
+        const int r_lo = col_ok ? max(0, (f.z0 - zb - g + 1) >> 1) : 32;
+        const int r_hi = min(31, (f.z1 - zb - g) >> 1);
 #pragma unroll 1
         for (int k0 = 0; k0 < 32; k0 += 16) {
             float tn[16];
@@ -381,7 +384,7 @@ __global__ __launch_bounds__(256) void integrateExact(typename S::Word* __restri
             for (int k = 0; k < 16; ++k) {
                 const int r = g + 2 * (k0 + k);
                 const float t = cls == kTileFree ? 1.0f : L[r * kLdsStride + j];
-                tn[k] = (col_ok && zb + r >= f.z0 && zb + r <= f.z1) ? t : __builtin_nanf("");
+                tn[k] = (k0 + k >= r_lo && k0 + k <= r_hi) ? t : __builtin_nanf("");
             }
             // Lanes with nothing to fold read out of range (no access), so the loads
             // carry no branch and issue back to back.
@@ -403,10 +406,10 @@ __global__ __launch_bounds__(256) void integrateExact(typename S::Word* __restri
                     const float w_old = S::weight(wd[k]);
                     const float t_old = S::tsdf(wd[k]);
                     newly = (w_old == 0.0f);
-                    const typename S::Word nw =
-                        S::pack((t_old * w_old + tn[k]) / (w_old + 1.0f), smin(w_old + 1.0f, p.max_w));
-                    BufWord<S>::store(nw, rs, lane_off, (unsigned)(k0 + k) * row_step);
-                    nf = !S::far(nw);
+                    const float t_new = (t_old * w_old + tn[k]) / (w_old + 1.0f);
+                    BufWord<S>::store(S::pack(t_new, smin(w_old + 1.0f, p.max_w)), rs, lane_off,
+                                      (unsigned)(k0 + k) * row_step);
+                    nf = !S::farPacked(t_new);  // the new weight is >= 1
                 }
                 n_upd += upd ? 1u : 0u;
                 n_new += newly ? 1u : 0u;

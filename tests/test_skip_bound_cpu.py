@@ -74,3 +74,25 @@ def test_far_corners_interpolate_far():
         assert np.isfinite(f).all()
         assert f.min() >= lim, (lerp.__name__, float(f.min()))
         assert f.max() <= np.float32(1.5)
+
+
+def test_fp16_packed_far_test_matches_rounding():
+    """integrate decides far-ness of a packed FP16 word from its float tsdf before the
+    binary16 rounding: the thresholds must give exactly the words whose half value lies
+    in [kFarTsdf, 1.5]."""
+    k = far_threshold()
+    src = open(os.path.join(ROOT, "azu_amd", "csrc", "common.hpp")).read()
+    lo_hi = re.findall(r"farPacked\(float t\) \{ return t >= ([0-9.]+)f && t <= ([0-9.]+)f; \}", src)
+    assert len(lo_hi) == 1, lo_hi
+    lo, hi = np.float32(float(lo_hi[0][0])), np.float32(float(lo_hi[0][1]))
+    # every float32 in a band around both thresholds, plus a random spread
+    def around(x, n):
+        b = np.array([x], np.float32).view(np.int32)[0]
+        return np.arange(b - n, b + n, dtype=np.int32).view(np.float32)
+    rng = np.random.default_rng(3)
+    t = np.concatenate([around(lo, 1 << 20), around(hi, 1 << 20),
+                        rng.uniform(0.9, 1.6, 1 << 20).astype(np.float32)])
+    h = t.astype(np.float16).astype(np.float32)  # round to nearest even
+    want = (h >= k) & (h <= np.float32(1.5))
+    got = (t >= lo) & (t <= hi)
+    assert np.array_equal(want, got)
