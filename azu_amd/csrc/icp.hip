@@ -748,11 +748,13 @@ __device__ void icpLastBlock(IcpState* __restrict__ st, const double* __restrict
         if (tid == 0) {
             icpSolve(st, sh.sum, &sh.lanes, pd, level, iter);
             const unsigned long long t_end = __builtin_amdgcn_s_memrealtime();
-            st->ticks_sum += t_sum - t_last;
-            st->ticks_solve += t_end - t_sum;
-            st->ticks_n += 1;
-            st->work_px += (unsigned long long)npx;
-            st->work_proj += (unsigned long long)sh.sum[kIcpSlotsF + 3];
+            // diagnostics: atomics whose result is unused (no round trip)
+            __hip_atomic_fetch_add(&st->ticks_sum, t_sum - t_last, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&st->ticks_solve, t_end - t_sum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&st->ticks_n, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&st->work_px, (unsigned long long)npx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&st->work_proj, (unsigned long long)sh.sum[kIcpSlotsF + 3], __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
         }
     }
 }
