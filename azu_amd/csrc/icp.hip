@@ -578,7 +578,8 @@ __device__ __forceinline__ void icpAccumulate(const IcpIterArgs& a, const PoseDe
 
 __device__ __forceinline__ F3 load3(const float* __restrict__ p, int i) { return F3{p[3 * i], p[3 * i + 1], p[3 * i + 2]}; }
 
-// Two-level arrival ticket: block b counts in at group b % 8 (one counter per 256 B,
+// Two-level arrival ticket (launches of more than kFlatTicket blocks; smaller ones count
+// in at the top counter directly): block b counts in at group b % 8 (one counter per 256 B,
 // so the groups' atomics go to different channels instead of queueing on one address),
 // the last block of each group counts in at the top counter, and the last of those is
 // the grid's last block. Each arrival is a release (the block's partials are visible
@@ -587,8 +588,16 @@ __device__ __forceinline__ F3 load3(const float* __restrict__ p, int i) { return
 // the next launch. Returns true in the grid's last block (thread 0 calls it).
 constexpr int kTicketGroups = 8;
 constexpr int kTicketStride = 64;  // uints: 256 B between counters
+constexpr int kFlatTicket = 128;   // launches of at most this many blocks use one counter
 
 __device__ bool arriveLast(unsigned* ticket, unsigned b, unsigned nb) {
+    if (nb <= (unsigned)kFlatTicket) {
+        // few blocks: one counter (one round trip instead of two)
+        const unsigned p = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (p != nb - 1) return false;
+        __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return true;
+    }
     const unsigned ng = nb < (unsigned)kTicketGroups ? nb : (unsigned)kTicketGroups;
     const unsigned g = b % ng;
     const unsigned in_group = (nb - g + ng - 1) / ng;
