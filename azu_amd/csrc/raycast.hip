@@ -78,7 +78,7 @@ struct VolBuf {
 // span (or the cell is outside the volume), so the sample is far, and the march takes
 // its far step there without reading it.
 struct RaySkip {
-    unsigned* dsafe;  // W*H float bit patterns (+inf between calls), or null
+    unsigned* dsafe;  // per 2 x 2 pixel cell, float bit patterns (+inf between calls), or null
     const int* ok;
 };
 
@@ -300,11 +300,12 @@ __global__ __launch_bounds__(256) void raycastKernel(Vol<S> V, VolParams p, Pose
     // camera so far away that the float spacing of t exceeds twice the step) the march
     // would never end, and the ray is left a miss.
     if (!(fabsf(t_far) + min_step > fabsf(t_far)) || !(fabsf(t_near) + min_step > fabsf(t_near))) t_far = -1.0f;
-    // The skip bound of this pixel; the slot is left at +inf for the next call.
+    // The skip bound of this pixel's 2 x 2 cell (the cell lies in this wave's 8 x 8 tile;
+    // its first pixel resets it to +inf at the end, after every lane has read it).
+    const int cell = (y >> 1) * ((W + 1) >> 1) + (x >> 1);
     float t_safe = -__builtin_inff();
     if (skip.dsafe) {
-        const float d = __uint_as_float(skip.dsafe[o]);
-        skip.dsafe[o] = 0x7f800000u;
+        const float d = __uint_as_float(skip.dsafe[cell]);
         if (*skip.ok) t_safe = d / f3norm(dir);
     }
     bool found = false;
@@ -423,6 +424,7 @@ __global__ __launch_bounds__(256) void raycastKernel(Vol<S> V, VolParams p, Pose
     no[1] = nrm.y;
     no[2] = nrm.z;
     if (out.t_hit) out.t_hit[o] = found ? t_hit : __builtin_nanf("");
+    if (skip.dsafe && ((x | y) & 1) == 0) skip.dsafe[cell] = 0x7f800000u;
 }
 
 template <class S>
@@ -465,8 +467,8 @@ void launchRay(const void* vox, const VolParams& p, const FixedDiv& dvs, const R
 // hold a sample (every sample lies at camera depth >= min_depth), so the corners there
 // and the box edges' crossings of the plane are projected; the perspective image of that
 // convex part is their hull, and the pixel rectangle around it, widened by one pixel,
-// takes the minimum of the box's distance from the camera (less 1e-5 of it), one pixel
-// per lane. A sample at parameter t is the point cam + dir t, at distance t |dir|.
+// takes the minimum of the box's distance from the camera (less 1e-5 of it) in every
+// 2 x 2 pixel cell it touches, one cell per lane. A sample at parameter t is the point cam + dir t, at distance t |dir|.
 // ok = 0 (take every sample) unless the pose is finite and invertible and min_depth > 0.
 __global__ __launch_bounds__(256) void brickSplat(const unsigned char* __restrict__ bricks, int nb, VolParams p,
                                                   PoseArg pa, Intr K, int W, int H, unsigned* __restrict__ dsafe,
@@ -551,8 +553,9 @@ __global__ __launch_bounds__(256) void brickSplat(const unsigned char* __restric
     const int y0 = (int)fmaxf(floorf(fmaxf(vmin, -4.0f)) - 1.0f, 0.0f);
     const int y1 = (int)fminf(ceilf(fminf(vmax, (float)H + 4.0f)) + 1.0f, (float)(H - 1));
     if (x0 > x1 || y0 > y1) return;
-    const int w = x1 - x0 + 1, area = w * (y1 - y0 + 1);
-    for (int k = lane; k < area; k += 64) atomicMin(&dsafe[(size_t)(y0 + k / w) * W + (x0 + k % w)], dlow);
+    const int cw = (W + 1) >> 1, cx0 = x0 >> 1, cy0 = y0 >> 1;
+    const int w = (x1 >> 1) - cx0 + 1, area = w * ((y1 >> 1) - cy0 + 1);
+    for (int k = lane; k < area; k += 64) atomicMin(&dsafe[(size_t)(cy0 + k / w) * cw + (cx0 + k % w)], dlow);
 }
 
 __global__ void fillWords(unsigned* __restrict__ p, size_t n, unsigned v) {
@@ -567,7 +570,8 @@ hipError_t launchSkipInit(unsigned* dsafe, size_t n, hipStream_t st) {
     return hipGetLastError();
 }
 
-// bricks: the volume's brick map, or null to take every sample; dsafe (W*H words, all
+// bricks: the volume's brick map, or null to take every sample; dsafe (>= one word per
+// 2 x 2 pixel cell, all
 // +inf on entry and on return) and ok (one int): the skip bounds' scratch.
 hipError_t launchRaycast(int format, bool exact, const void* vox, const VolParams& p, const FixedDiv& dvs,
                          const unsigned char* bricks, unsigned* dsafe, int* ok, const PoseArg& pa, Intr K, int W,
