@@ -33,7 +33,6 @@ hipError_t launchLive(bool mm, const void* src, int W, int H, Intr K, float min_
 size_t icpStateBytes();
 int icpBlocksFor(int W, int H);
 size_t icpPartialsBytes(int maxBlocks);
-size_t icpTicketBytes();
 hipError_t launchIcpInit(void* st, const PoseArg& est, const PoseArg& ref, hipStream_t s);
 hipError_t launchPredict(const float* cur, float* last, float* pred, hipStream_t s);
 hipError_t launchIcpClearTicks(void* st, hipStream_t s);
@@ -52,8 +51,8 @@ hipError_t launchDepthLoss(int format, const void* vox, const VolParams& p, cons
                            double* partials, double* out8, hipStream_t st);
 void icpTicksRead(const void* hostCopy, unsigned long long* ticks);
 hipError_t launchIcpIter(const float* lv, const float* ln, int W, int H, const float* mv, const float* mn, int MW,
-                         int MH, Intr Km, float angle_cos, float dist_sq, int level, int iter, void* st,
-                         double* partials, unsigned* ticket, hipStream_t s);
+                         int MH, Intr Km, float angle_cos, float dist_sq, int level, int iter, int prev_level,
+                         int prev_iter, int prev_W, int prev_H, int k, void* st, double* partials, hipStream_t s);
 void icpStateRead(const void* hostCopy, float* pose, float* information, float* error, float* final_step,
                   int* inliers, int* converged, int* counters, int* iterations);
 hipError_t launchDownsample(int W, int H, float* const d[3], float* const v[3], float* const n[3], hipStream_t st);
@@ -231,7 +230,6 @@ struct azu_tracker {
     float* n[3] = {nullptr, nullptr, nullptr};
     void* state = nullptr;
     double* partials = nullptr;
-    unsigned* ticket = nullptr;
     void* host_state = nullptr;  // pinned
     float* motion = nullptr;     // device: previous pose [16], predicted pose [16]
     bool tracked = false;
@@ -712,7 +710,6 @@ int azu_tracker_destroy(azu_tracker* t) {
     }
     if (t->state) (void)hipFree(t->state);
     if (t->partials) (void)hipFree(t->partials);
-    if (t->ticket) (void)hipFree(t->ticket);
     if (t->host_state) (void)hipHostFree(t->host_state);
     if (t->motion) (void)hipFree(t->motion);
     delete t;
@@ -744,15 +741,13 @@ int azu_tracker_create(int W, int H, const float K[4], const azu_icp_params* p, 
     }
     ok = ok && hipMalloc(&t->state, icpStateBytes()) == hipSuccess;
     ok = ok && hipMalloc((void**)&t->partials, icpPartialsBytes(1024)) == hipSuccess;
-    ok = ok && hipMalloc((void**)&t->ticket, icpTicketBytes()) == hipSuccess;
     ok = ok && hipHostMalloc(&t->host_state, icpStateBytes(), hipHostMallocDefault) == hipSuccess;
     ok = ok && hipMalloc((void**)&t->motion, 32 * sizeof(float)) == hipSuccess;
     if (!ok) {
         azu_tracker_destroy(t);
         return fail(AZU_ERR_ALLOC, "tracker allocation failed");
     }
-    if (hipMemset(t->ticket, 0, icpTicketBytes()) != hipSuccess ||
-        hipMemset(t->state, 0, icpStateBytes()) != hipSuccess) {
+    if (hipMemset(t->state, 0, icpStateBytes()) != hipSuccess) {
         azu_tracker_destroy(t);
         return fail(AZU_ERR_HIP, "tracker initialisation failed");
     }
@@ -818,14 +813,26 @@ static int trackImpl(azu_tracker* t, const float* mv, const float* mn, int MW, i
         ProfScope ps("icp_init", st);
         AZU_HIP_TRY(launchIcpInit(t->state, est, ref, st), "ICP init launch");
     }
+    // Launch k solves iteration k - 1 and runs the pass of iteration k; a last launch
+    // solves the final iteration.
+    int k = 0, pl = -1, pi = 0;
     for (int level = 2; level >= 0; --level) {
         const int iters = t->params.max_iterations[level];
-        for (int it = 0; it < iters; ++it) {
+        for (int it = 0; it < iters; ++it, ++k) {
             ProfScope ps("icp_iter", st);
             AZU_HIP_TRY(launchIcpIter(t->v[level], t->n[level], t->lw[level], t->lh[level], mv, mn, MW, MH, Km,
-                                      angle_cos, dist_sq, level, it, t->state, t->partials, t->ticket, st),
+                                      angle_cos, dist_sq, level, it, pl, pi, pl >= 0 ? t->lw[pl] : 0,
+                                      pl >= 0 ? t->lh[pl] : 0, k, t->state, t->partials, st),
                         "ICP iteration launch");
+            pl = level;
+            pi = it;
         }
+    }
+    if (pl >= 0) {
+        ProfScope ps("icp_iter", st);
+        AZU_HIP_TRY(launchIcpIter(nullptr, nullptr, 0, 0, mv, mn, MW, MH, Km, angle_cos, dist_sq, -1, 0, pl, pi,
+                                  t->lw[pl], t->lh[pl], k, t->state, t->partials, st),
+                    "ICP final solve launch");
     }
     t->tracked = true;
     return AZU_OK;

@@ -2,21 +2,23 @@
 // trackLevel / buildLinearSystem, src/tracking/ICPTracker.cpp:40-354, with the
 // constants and predicates of include/tracking/ICPShared.h:34-110).
 //
-// One kernel launch per ICP iteration, no host round trip:
-//   1. every thread walks its live pixels (grid-stride), associates each with the model
-//      image, builds J = [R^T n_m ; v x R^T n_m], the Huber weight and loss, and
-//      accumulates the 21 upper-triangle A terms, 6 b terms and the residual in FP32
-//      with Kahan (TwoSum) compensation, plus 6 integer counters;
-//   2. the block folds its lanes with DPP wave reductions and LDS, still as
-//      compensated FP32 pairs, and writes one FP64 partial per slot;
-//   3. the last block to finish (atomic ticket, agent-scope fences) sums the partials
-//      in FP64 in a fixed order (deterministic), issuing all of its loads at once, and
-//      one lane runs the solve: adaptive damping from the extreme eigenvalues, FP64
+// One kernel launch per ICP iteration (and a final solve-only launch), no host round
+// trip. Launch k:
+//   1. solves iteration k - 1 in every block: the FP64 partials that launch k - 1 wrote
+//      are summed in a fixed order (deterministic, so every block gets the same sums),
+//      and one lane runs the solve: adaptive damping from the extreme eigenvalues, FP64
 //      Cholesky of A + lambda I, step caps, Rodrigues update, pose * delta,
-//      re-projection onto SO(3), convergence test.
-// The tracker state (pose, per-level result, stop flag) lives in device memory, so
-// all 19 iterations of a frame are enqueued back to back; once a level has stopped,
-// its remaining launches return at once.
+//      re-projection onto SO(3), convergence test. Block 0 records the result. The
+//      kernel boundary orders the partials, so no block waits for another;
+//   2. then, unless the level has stopped, every thread walks its live pixels, associates
+//      each with the model image, builds J = [R^T n_m ; v x R^T n_m], the Huber weight
+//      and loss, and accumulates the 21 upper-triangle A terms, 6 b terms and the
+//      residual in FP32 with Kahan (TwoSum) compensation, plus 6 integer counters; the
+//      block folds its lanes through LDS, still as compensated pairs, and writes one
+//      FP64 partial per slot.
+// The tracker state (pose, per-level result, stop flag) lives in device memory, so all
+// launches of a frame are enqueued back to back; once a level has stopped, its remaining
+// launches return at once.
 #include "common.hpp"
 
 namespace azu {
@@ -25,7 +27,7 @@ constexpr int kIcpSlotsF = 28;  // 21 A + 6 b + residual
 constexpr int kIcpSlotsI = 6;   // inliers, valid_live, valid_model, projected, dist_f, angle_f
 constexpr int kIcpSlots = kIcpSlotsF + kIcpSlotsI;
 constexpr int kIcpMaxBlocks = 256;
-constexpr int kSumGroups = 7;  // last block: 7 threads per slot (7 * 34 = 238 <= 256)
+constexpr int kSumGroups = 7;  // cross-block sum: 7 threads per slot (7 * 34 = 238 <= 256)
 
 // Pose terms every thread of an iteration uses: the current pose (camera to world) and
 // the transform from the live camera into the reference camera. Computed once per
@@ -52,8 +54,12 @@ struct IcpState {
     // that reached it: cross-block sum, solve; and the number of such launches. Work of
     // the same launches: interior live pixels visited, in-bounds model projections.
     unsigned long long ticks_sum, ticks_solve, ticks_n, work_px, work_proj;
-    PoseDerived next;  // the pose terms of pose / ref / iterations above
-    int stop_level;    // level whose iteration loop has ended, -1 while none
+    // Per launch parity p: launch k reads it[k & 1] and its block 0 writes it[1 - (k & 1)].
+    struct Slot {
+        PoseDerived pd;  // the pose terms the launch's pass uses (after its solve)
+        int pending;     // the launch before ran a pass: its partials await a solve
+    } it[2];
+    int stop_level;  // level whose iteration loop has ended, -1 while none
 };
 
 struct IcpIterArgs {
@@ -330,11 +336,14 @@ __device__ PoseDerived derivePose(const float* pose, const float* ref, int itera
 }
 
 // One solve + pose update (trackLevel's loop body, ICPTracker.cpp:79-130), lane 0 of
-// the last block once solveLane has run on lanes 0-2. sums: 28 FP64 slots (21 A upper,
-// 6 b, residual) + 6 counters.
-__device__ void icpSolve(IcpState* st, const double* sums, const SolveLanes* sl, const PoseDerived& pd, int level,
-                         int iter) {
-    if (iter == 0) {
+// wave 0 once solveLane has run on lanes 0-2; every block of a launch runs it on the same
+// sums and gets the same result, and the writer (block 0) also records it in the state.
+// sums: 28 FP64 slots (21 A upper, 6 b, residual) + 6 counters. out: the pose terms after
+// the step (pd when there is none); returns true when the level's loop ends.
+__device__ bool icpSolve(IcpState* st, bool writer, const double* sums, const SolveLanes* sl, const PoseDerived& pd,
+                         int level, int iter, PoseDerived& out) {
+    out = pd;
+    if (writer && iter == 0) {
         st->error = 0.0f;
         st->inliers = 0;
         st->converged = 0;
@@ -342,15 +351,17 @@ __device__ void icpSolve(IcpState* st, const double* sums, const SolveLanes* sl,
         for (int i = 0; i < 36; ++i) st->information[i] = 0.0f;
     }
     const int inliers = (int)sums[kIcpSlotsF + 0];
-    for (int i = 0; i < 5; ++i) st->counters[i] = (int)sums[kIcpSlotsF + 1 + i];
+    if (writer)
+        for (int i = 0; i < 5; ++i) st->counters[i] = (int)sums[kIcpSlotsF + 1 + i];
     if (inliers < 10) {  // covers buildLinearSystem's "no inlier" failure too
-        st->stop_level = level;
-        return;
+        if (writer) st->stop_level = level;
+        return true;
     }
     double A[36], b[6];
     buildSystem(sums, A, b);
     const double residual = sums[27];
-    for (int i = 0; i < 36; ++i) st->information[i] = (float)A[i];
+    if (writer)
+        for (int i = 0; i < 36; ++i) st->information[i] = (float)A[i];
     int pick = 1;  // the level's base lambda
     if (!sl->finite) {
         pick = 2;
@@ -361,8 +372,8 @@ __device__ void icpSolve(IcpState* st, const double* sums, const SolveLanes* sl,
         if (!isfinite(mn) || !isfinite(mx) || mn < kMin || mx > kCond * mn) pick = 2;
     }
     if (!sl->ok[pick]) {
-        st->stop_level = level;
-        return;
+        if (writer) st->stop_level = level;
+        return true;
     }
     double xd[6];
     for (int i = 0; i < 6; ++i) xd[i] = sl->x[pick][i];
@@ -373,14 +384,14 @@ __device__ void icpSolve(IcpState* st, const double* sums, const SolveLanes* sl,
         fin = fin && isfinite(x[i]);
     }
     if (!fin) {
-        st->stop_level = level;
-        return;
+        if (writer) st->stop_level = level;
+        return true;
     }
     const float tn = sqrtf(sum3(x[0] * x[0], x[1] * x[1], x[2] * x[2]));
     const float rn = sqrtf(sum3(x[3] * x[3], x[4] * x[4], x[5] * x[5]));
     if (tn > 0.2f || rn > 0.5f) {
-        st->stop_level = level;
-        return;
+        if (writer) st->stop_level = level;
+        return true;
     }
     float sq = 0.0f;
     for (int i = 0; i < 6; ++i) sq += x[i] * x[i];
@@ -410,8 +421,7 @@ __device__ void icpSolve(IcpState* st, const double* sums, const SolveLanes* sl,
         Rd[2][0] = -ry;
         Rd[2][1] = rx;
     }
-    // The pose this launch started from (read into LDS at its start; only this block
-    // writes it, after every block has read it).
+    // The pose the solved iteration started from.
     float P[3][3], t[3] = {pd.t_cw.x, pd.t_cw.y, pd.t_cw.z};
     for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) P[i][j] = pd.R_cw.m[i][j];
@@ -423,25 +433,30 @@ __device__ void icpSolve(IcpState* st, const double* sums, const SolveLanes* sl,
     }
     double Rp[9];
     projectSO3(Md, Rp);
-    float np[16];
-    for (int i = 0; i < 16; ++i) np[i] = st->pose[i];  // the bottom row stays as it was
+    float np[16] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1};
+    if (writer)
+        for (int i = 0; i < 16; ++i) np[i] = st->pose[i];  // the bottom row stays as it was
     for (int i = 0; i < 3; ++i) {
         for (int j = 0; j < 3; ++j) np[j * 4 + i] = (float)Rp[i * 3 + j];
         np[12 + i] = nt[i];
     }
-    for (int i = 0; i < 16; ++i) st->pose[i] = np[i];
-    st->final_step = step_norm;
-    st->last_final_step = step_norm;
-    st->iterations = pd.iterations + (1 << (8 * level));
     float rf[16];
-    for (int i = 0; i < 16; ++i) rf[i] = st->ref[i];
-    st->next = derivePose(np, rf, pd.iterations + (1 << (8 * level)));
-    st->error = (float)(residual / (double)(inliers > 1 ? inliers : 1));
-    st->inliers = inliers;
-    if (step_norm < 5e-5f) {
-        st->converged = 1;
-        st->stop_level = level;
+    for (int i = 0; i < 16; ++i) rf[i] = st->ref[i];  // constant over the track
+    out = derivePose(np, rf, pd.iterations + (1 << (8 * level)));
+    const bool converged = step_norm < 5e-5f;
+    if (writer) {
+        for (int i = 0; i < 16; ++i) st->pose[i] = np[i];
+        st->final_step = step_norm;
+        st->last_final_step = step_norm;
+        st->iterations = pd.iterations + (1 << (8 * level));
+        st->error = (float)(residual / (double)(inliers > 1 ? inliers : 1));
+        st->inliers = inliers;
+        if (converged) {
+            st->converged = 1;
+            st->stop_level = level;
+        }
     }
+    return converged;
 }
 
 __global__ void icpInit(IcpState* st, PoseArg est, PoseArg ref) {
@@ -465,7 +480,10 @@ __global__ void icpInit(IcpState* st, PoseArg est, PoseArg ref) {
     for (int i = 0; i < 5; ++i) st->counters[i] = 0;
     st->stop_level = -1;
     st->iterations = 0;
-    st->next = derivePose(e, r, 0);
+    for (int p = 0; p < 2; ++p) {
+        st->it[p].pd = derivePose(e, r, 0);
+        st->it[p].pending = 0;
+    }
 }
 
 __global__ void icpClearTicks(IcpState* st) {
@@ -578,40 +596,6 @@ __device__ __forceinline__ void icpAccumulate(const IcpIterArgs& a, const PoseDe
 
 __device__ __forceinline__ F3 load3(const float* __restrict__ p, int i) { return F3{p[3 * i], p[3 * i + 1], p[3 * i + 2]}; }
 
-// Two-level arrival ticket (launches of more than kFlatTicket blocks; smaller ones count
-// in at the top counter directly): block b counts in at group b % 8 (one counter per 256 B,
-// so the groups' atomics go to different channels instead of queueing on one address),
-// the last block of each group counts in at the top counter, and the last of those is
-// the grid's last block. Each arrival is a release (the block's partials are visible
-// before its count), a group's last block acquires before it releases onward, and the
-// grid's last block acquires; counters are reset by the block that completes them, for
-// the next launch. Returns true in the grid's last block (thread 0 calls it).
-constexpr int kTicketGroups = 8;
-constexpr int kTicketStride = 64;  // uints: 256 B between counters
-constexpr int kFlatTicket = 128;   // launches of at most this many blocks use one counter
-
-__device__ bool arriveLast(unsigned* ticket, unsigned b, unsigned nb) {
-    if (nb <= (unsigned)kFlatTicket) {
-        // few blocks: one counter (one round trip instead of two)
-        const unsigned p = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (p != nb - 1) return false;
-        __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        return true;
-    }
-    const unsigned ng = nb < (unsigned)kTicketGroups ? nb : (unsigned)kTicketGroups;
-    const unsigned g = b % ng;
-    const unsigned in_group = (nb - g + ng - 1) / ng;
-    unsigned* gc = ticket + (1 + g) * kTicketStride;
-    const unsigned p = __hip_atomic_fetch_add(gc, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    if (p != in_group - 1) return false;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    __hip_atomic_store(gc, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned q = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    if (q != ng - 1) return false;
-    __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return true;
-}
-
 constexpr int kPxBatch = 5;   // pixels whose reads one thread issues together
 constexpr int kFoldGroups = 9;  // block fold: threads per system slot (28 * 9 = 252 <= 256)
 
@@ -632,7 +616,8 @@ struct IcpShared {
     double grp[kSumGroups][kIcpSlots];
     double sum[kIcpSlots];
     SolveLanes lanes;
-    int last;
+    PoseDerived pd;
+    int stop;
 };
 
 
@@ -719,14 +704,15 @@ __device__ int icpBlockPass(const IcpIterArgs& a, const PoseDerived& pd, double*
     return npx;
 }
 
-// The grid's last block: FP64 cross-block sum, then the solve and state update. Thread
-// (g, s) sums slot s over blocks g, g + 7, g + 14, ... with all of its loads issued
-// before the first add; the seven group sums are then added in group order. The order
-// is fixed, so the result is too.
-__device__ void icpLastBlock(IcpState* __restrict__ st, const double* __restrict__ partials, IcpShared& sh,
-                             const PoseDerived& pd, int level, int iter, int npx, int nblk,
-                             unsigned long long t_last) {
+// The previous iteration's solve, in every block: FP64 cross-block sum of its partials,
+// then the solve. Thread (g, s) sums slot s over blocks g, g + 7, g + 14, ... with all of
+// its loads issued before the first add; the seven group sums are then added in group
+// order. The order is fixed, so every block gets the same sums, and the same solve.
+// Returns true when the level's loop ends; the new pose terms are left in sh.pd.
+__device__ bool blockSolve(IcpState* __restrict__ st, const double* __restrict__ partials, IcpShared& sh,
+                           const PoseDerived& pd, int level, int iter, int npx, int nblk, bool writer) {
     const int tid = threadIdx.x;
+    const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
     const unsigned nb = (unsigned)nblk;
     if (tid < kSumGroups * kIcpSlots) {
         const int s = tid % kIcpSlots, g = tid / kIcpSlots;
@@ -755,35 +741,58 @@ __device__ void icpLastBlock(IcpState* __restrict__ st, const double* __restrict
         if (tid < 3 && (int)sh.sum[kIcpSlotsF] >= 10) solveLane(sh.sum, level, tid, &sh.lanes);
         waveSync();
         if (tid == 0) {
-            icpSolve(st, sh.sum, &sh.lanes, pd, level, iter);
-            const unsigned long long t_end = __builtin_amdgcn_s_memrealtime();
-            // diagnostics: atomics whose result is unused (no round trip)
-            __hip_atomic_fetch_add(&st->ticks_sum, t_sum - t_last, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_fetch_add(&st->ticks_solve, t_end - t_sum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_fetch_add(&st->ticks_n, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_fetch_add(&st->work_px, (unsigned long long)npx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_fetch_add(&st->work_proj, (unsigned long long)sh.sum[kIcpSlotsF + 3], __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
+            PoseDerived out;
+            sh.stop = icpSolve(st, writer, sh.sum, &sh.lanes, pd, level, iter, out) ? 1 : 0;
+            sh.pd = out;
+            if (writer) {
+                // diagnostics: atomics whose result is unused (no round trip)
+                const unsigned long long t_end = __builtin_amdgcn_s_memrealtime();
+                __hip_atomic_fetch_add(&st->ticks_sum, t_sum - t0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_fetch_add(&st->ticks_solve, t_end - t_sum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_fetch_add(&st->ticks_n, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_fetch_add(&st->work_px, (unsigned long long)npx, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_fetch_add(&st->work_proj, (unsigned long long)sh.sum[kIcpSlotsF + 3], __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+            }
         }
     }
+    __syncthreads();
+    return sh.stop != 0;
 }
 
-// One ICP iteration per launch.
-__global__ __launch_bounds__(256) void icpIter(IcpIterArgs a, IcpState* __restrict__ st,
-                                               double* __restrict__ partials, unsigned* __restrict__ ticket) {
+// One launch per ICP iteration, plus a final solve-only launch: first the previous
+// iteration's solve (every block, from the partials the previous launch wrote; the kernel
+// boundary orders them, so no block waits on another), then this iteration's pixel pass
+// into fresh partials. prev.level < 0: nothing to solve; a.level < 0: no pass.
+struct IcpPrev {
+    int level, iter, npx, nblk;
+};
+
+__global__ __launch_bounds__(256) void icpIter(IcpIterArgs a, IcpPrev prev, IcpState* __restrict__ st,
+                                               const double* __restrict__ part_prev, double* __restrict__ part_cur,
+                                               int par) {
     __shared__ IcpShared sh;
-    // The stop flag and the pose terms, read together (uniform over the grid; written
-    // only by a last block, after every block of its launch has read them).
-    const int stop = st->stop_level;
-    const PoseDerived pd = st->next;
-    if (stop == a.level) return;
     const int tid = threadIdx.x;
-    const int npx = icpBlockPass(a, pd, partials, sh, gridDim.x);
-    __syncthreads();
-    if (tid == 0) sh.last = arriveLast(ticket, blockIdx.x, gridDim.x) ? 1 : 0;
-    __syncthreads();
-    if (!sh.last) return;
-    icpLastBlock(st, partials, sh, pd, a.level, a.iter, npx, gridDim.x, __builtin_amdgcn_s_memrealtime());
+    const bool writer = blockIdx.x == 0;
+    // Read at the start: the stop level (block 0 of this launch may set it, to a value
+    // every block also reaches by its own solve) and this launch's slot (written by the
+    // launch before; block 0 writes the other one).
+    const int stop = st->stop_level;
+    const int pending = st->it[par].pending;
+    PoseDerived pd = st->it[par].pd;
+    bool stopped = false;
+    if (pending && prev.level >= 0) {
+        stopped = blockSolve(st, part_prev, sh, pd, prev.level, prev.iter, prev.npx, prev.nblk, writer);
+        pd = sh.pd;
+    }
+    const bool pass = a.level >= 0 && stop != a.level && !(stopped && prev.level == a.level);
+    if (writer && tid == 0) {
+        st->it[1 - par].pd = pd;
+        st->it[1 - par].pending = pass ? 1 : 0;
+    }
+    if (!pass) return;
+    icpBlockPass(a, pd, part_cur, sh, gridDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -791,8 +800,8 @@ __global__ __launch_bounds__(256) void icpIter(IcpIterArgs a, IcpState* __restri
 // ---------------------------------------------------------------------------
 size_t icpStateBytes() { return sizeof(IcpState); }
 
-size_t icpPartialsBytes(int) { return sizeof(double) * kIcpSlots * (size_t)kIcpMaxBlocks; }
-size_t icpTicketBytes() { return sizeof(unsigned) * kTicketStride * (1 + kTicketGroups); }
+// Two halves (the launch's own and the previous launch's partials).
+size_t icpPartialsBytes(int) { return 2 * sizeof(double) * kIcpSlots * (size_t)kIcpMaxBlocks; }
 
 hipError_t launchIcpInit(void* st, const PoseArg& est, const PoseArg& ref, hipStream_t s) {
     hipLaunchKernelGGL(icpInit, dim3(1), dim3(64), 0, s, (IcpState*)st, est, ref);
@@ -804,9 +813,11 @@ hipError_t launchPredict(const float* cur, float* last, float* pred, hipStream_t
     return hipGetLastError();
 }
 
+// Launch k of a track (k = 0, 1, ...): level < 0 is the final, solve-only launch (one
+// block); prev_level < 0: no previous iteration. partials: icpPartialsBytes.
 hipError_t launchIcpIter(const float* lv, const float* ln, int W, int H, const float* mv, const float* mn, int MW,
-                         int MH, Intr Km, float angle_cos, float dist_sq, int level, int iter, void* st,
-                         double* partials, unsigned* ticket, hipStream_t s) {
+                         int MH, Intr Km, float angle_cos, float dist_sq, int level, int iter, int prev_level,
+                         int prev_iter, int prev_W, int prev_H, int k, void* st, double* partials, hipStream_t s) {
     IcpIterArgs a;
     a.lv = lv;
     a.ln = ln;
@@ -821,7 +832,16 @@ hipError_t launchIcpIter(const float* lv, const float* ln, int W, int H, const f
     a.dist_sq = dist_sq;
     a.level = level;
     a.iter = iter;
-    hipLaunchKernelGGL(icpIter, dim3(icpBlocksFor(W, H)), dim3(256), 0, s, a, (IcpState*)st, partials, ticket);
+    IcpPrev prev{prev_level, prev_iter, 0, 1};
+    if (prev_level >= 0) {
+        prev.npx = (prev_W > 2 && prev_H > 2) ? (prev_W - 2) * (prev_H - 2) : 0;
+        prev.nblk = icpBlocksFor(prev_W, prev_H);
+    }
+    const size_t half = (size_t)kIcpSlots * kIcpMaxBlocks;
+    double* cur = partials + (size_t)(k & 1) * half;
+    const double* old = partials + (size_t)((k + 1) & 1) * half;
+    const unsigned blocks = level >= 0 ? (unsigned)icpBlocksFor(W, H) : 1u;
+    hipLaunchKernelGGL(icpIter, dim3(blocks), dim3(256), 0, s, a, prev, (IcpState*)st, old, cur, k & 1);
     return hipGetLastError();
 }
 
