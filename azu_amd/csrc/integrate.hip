@@ -316,10 +316,26 @@ __global__ __launch_bounds__(256) void integrateExact(typename S::Word* __restri
     const float cbound = fabsf(f.t_wc.x) + fabsf(f.t_wc.y) + fabsf(f.t_wc.z) +
                          fabsf(p.origin[0]) + fabsf(p.origin[1]) + fabsf(p.origin[2]) + 3.0f * (float)p.res * vs;
 
-    for (int xc = f.x0 & ~(kChunk - 1); xc <= f.x1; xc += kChunk) {
+    // Classify every tile of the row at once, lane t taking tile t (at most 64 tiles: res
+    // <= 2048), so the depth-bounds reads of all tiles overlap; the classes come back as
+    // two wave masks.
+    const int xstart = f.x0 & ~(kChunk - 1);
+    unsigned long long cls_lo, cls_hi;
+    {
+        int my = kTileCull;
+        const int xt = xstart + lane * kChunk;
+        if (xt <= f.x1) {
+            const int js = max(f.x0 - xt, 0), n = min(kChunk, f.x1 - xt + 1);
+            my = tileClass(f, p, K, W, H, max_d, xt + js, xt + n - 1, y, za, ze, bnd, cbound);
+        }
+        cls_lo = __ballot((my & 1) != 0);
+        cls_hi = __ballot((my & 2) != 0);
+    }
+    for (int xc = xstart; xc <= f.x1; xc += kChunk) {
         // columns [js, n) of the tile are in the frustum box
         const int js = max(f.x0 - xc, 0), n = min(kChunk, f.x1 - xc + 1);
-        const int cls = tileClass(f, p, K, W, H, max_d, xc + js, xc + n - 1, y, za, ze, bnd, cbound);
+        const int ti = (xc - xstart) / kChunk;
+        const int cls = (int)((cls_lo >> ti) & 1ull) | ((int)((cls_hi >> ti) & 1ull) << 1);
         if (cls != kTileExact) {
             // The row's camera coordinate still advances step by step.
             for (int j = js; j < n; ++j) c = f3add(c, f.step);
