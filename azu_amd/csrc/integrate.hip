@@ -152,17 +152,23 @@ __global__ __launch_bounds__(256) void depthBoundsKernel(const float* __restrict
 }
 
 // Bounds over the pixel rectangle [u0, u1] x [v0, v1] (inside the image), from the
-// coarsest level needed to cover it with at most 2 x 2 cells; false when the rectangle
-// is wider than a level-kBoundLevels cell.
+// coarsest level needed to cover it with at most 2 x 2 cells; a rectangle wider than a
+// level-kBoundLevels cell takes up to 4 x 4 cells of that level; false beyond.
 __device__ __forceinline__ bool boundsOver(const DepthBounds& b, int u0, int u1, int v0, int v1, float2& r) {
     const int ext = max(u1 - u0, v1 - v0) + 1;
     int l = 1;
-    while ((1 << l) < ext) ++l;
-    if (l > kBoundLevels) return false;
+    while ((1 << l) < ext && l < kBoundLevels) ++l;
     const int cx0 = u0 >> l, cx1 = u1 >> l, cy0 = v0 >> l, cy1 = v1 >> l;
     const float2* L = b.lvl + b.off[l];
-    r = boundsJoin(boundsJoin(L[cy0 * b.w[l] + cx0], L[cy0 * b.w[l] + cx1]),
-                   boundsJoin(L[cy1 * b.w[l] + cx0], L[cy1 * b.w[l] + cx1]));
+    if (cx1 - cx0 < 2 && cy1 - cy0 < 2) {
+        r = boundsJoin(boundsJoin(L[cy0 * b.w[l] + cx0], L[cy0 * b.w[l] + cx1]),
+                       boundsJoin(L[cy1 * b.w[l] + cx0], L[cy1 * b.w[l] + cx1]));
+        return true;
+    }
+    if (cx1 - cx0 > 3 || cy1 - cy0 > 3) return false;
+    r = L[cy0 * b.w[l] + cx0];
+    for (int cy = cy0; cy <= cy1; ++cy)
+        for (int cx = cx0; cx <= cx1; ++cx) r = boundsJoin(r, L[cy * b.w[l] + cx]);
     return true;
 }
 
