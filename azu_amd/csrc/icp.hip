@@ -553,7 +553,7 @@ __device__ __forceinline__ int icpProject(const IcpIterArgs& a, const PoseDerive
 }
 
 __device__ __forceinline__ void icpAccumulate(const IcpIterArgs& a, const PoseDerived& pd, F3 lv, F3 mv, F3 mn,
-                                              F3 ln, float* hi, float* lo, int* cnt) {
+                                              F3 ln, float* acc, int* cnt) {
     if (!(f3finite(mv) && f3dot(mv, mv) > 1e-12f) || !(f3finite(mn) && f3dot(mn, mn) > 0.9f)) return;
     cnt[2]++;
     const F3 vlw = f3add(mulv(pd.R_cw, lv), pd.t_cw);
@@ -579,18 +579,22 @@ __device__ __forceinline__ void icpAccumulate(const IcpIterArgs& a, const PoseDe
     const float w = (ae <= 0.02f) ? 1.0f : 0.02f / ae;
     const float we = err * w;
     const float loss = (ae <= 0.02f) ? ae * ae : 2.0f * 0.02f * ae - 0.02f * 0.02f;
+    // A thread folds a handful of pixels (at most kPxBatch per level-0 pass), so plain
+    // fused accumulation keeps its sums within a few float roundings; the long sums run
+    // compensated in the block fold and in FP64 across blocks.
     int k = 0;
 #pragma unroll
     for (int i = 0; i < 6; ++i) {
+        const float wj = w * J[i];
 #pragma unroll
         for (int j = i; j < 6; ++j) {
-            twoSum(hi[k], lo[k], w * J[i] * J[j], 0.0f);
+            acc[k] = __builtin_fmaf(wj, J[j], acc[k]);
             ++k;
         }
     }
 #pragma unroll
-    for (int i = 0; i < 6; ++i) twoSum(hi[21 + i], lo[21 + i], -(J[i] * we), 0.0f);
-    twoSum(hi[27], lo[27], loss, 0.0f);
+    for (int i = 0; i < 6; ++i) acc[21 + i] = __builtin_fmaf(-J[i], we, acc[21 + i]);
+    acc[27] += loss;
     cnt[0]++;
 }
 
@@ -609,8 +613,31 @@ __host__ __device__ int icpBlocksFor(int W, int H) {
     return (int)b;
 }
 
+// One batch of a thread's live pixels: level indices (-1 past the level's end) and the
+// live vertices and normals there.
+struct PassBatch {
+    int idx[kPxBatch];
+    F3 lv[kPxBatch], ln[kPxBatch];
+};
+
+// Pixels p0, p0 + stride, ... of the level's interior; every read is issued before any
+// use.
+__device__ __forceinline__ void loadBatch(const IcpIterArgs& a, int p0, int stride, PassBatch& pb) {
+    const int iw = a.W - 2, ih = a.H - 2;
+    const int npx = (iw > 0 && ih > 0) ? iw * ih : 0;
+#pragma unroll
+    for (int b = 0; b < kPxBatch; ++b) {
+        const int p = p0 + b * stride;
+        pb.idx[b] = p < npx ? (1 + p / iw) * a.W + 1 + p % iw : -1;
+    }
+#pragma unroll
+    for (int b = 0; b < kPxBatch; ++b) pb.lv[b] = load3(a.lv, pb.idx[b] < 0 ? 0 : pb.idx[b]);
+#pragma unroll
+    for (int b = 0; b < kPxBatch; ++b) pb.ln[b] = load3(a.ln, pb.idx[b] < 0 ? 0 : pb.idx[b]);
+}
+
 struct IcpShared {
-    float red[2 * kIcpSlotsF][256];  // every thread's compensated pairs, slot-major
+    float red[kIcpSlotsF][256];  // every thread's sums, slot-major
     double part[kIcpSlotsF][kFoldGroups];
     int cnt[4][kIcpSlotsI];
     double grp[kSumGroups][kIcpSlots];
@@ -622,58 +649,45 @@ struct IcpShared {
 
 
 // One block's share of an iteration: the pixel pass over its pixels, the block fold and
-// its FP64 partials (written by wave 0). The level is split over nblk blocks (blocks
-// from nblk on have no pixels and write nothing). Returns the level's interior pixel
-// count.
-__device__ int icpBlockPass(const IcpIterArgs& a, const PoseDerived& pd, double* __restrict__ partials,
-                            IcpShared& sh, int nblk) {
+// its FP64 partials (written by wave 0). The level is split over the launch's blocks;
+// first: the thread's first batch (loadBatch at p0 = blockIdx.x * 256 + tid, stride =
+// gridDim.x * 256), already requested.
+__device__ void icpBlockPass(const IcpIterArgs& a, const PoseDerived& pd, double* __restrict__ partials,
+                             IcpShared& sh, const PassBatch& first) {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    float hi[kIcpSlotsF], lo[kIcpSlotsF];
+    float acc[kIcpSlotsF];
 #pragma unroll
-    for (int i = 0; i < kIcpSlotsF; ++i) {
-        hi[i] = 0.0f;
-        lo[i] = 0.0f;
-    }
+    for (int i = 0; i < kIcpSlotsF; ++i) acc[i] = 0.0f;
     int cnt[kIcpSlotsI] = {0, 0, 0, 0, 0, 0};
     const int iw = a.W - 2, ih = a.H - 2;
     const int npx = (iw > 0 && ih > 0) ? iw * ih : 0;
-    if ((int)blockIdx.x >= nblk) return npx;
-    const int stride = nblk * 256;
-    // Each thread visits pixels p0, p0 + stride, ... in that order; per batch, all live
-    // vertex reads, then all model / live-normal reads, are issued before their use.
-    for (int p0 = blockIdx.x * 256 + tid; p0 < npx; p0 += kPxBatch * stride) {
-        int idx[kPxBatch], midx[kPxBatch];
-        F3 lv[kPxBatch];
+    const int stride = (int)gridDim.x * 256;
+    // Each thread visits pixels p0, p0 + stride, ... in that order, kPxBatch at a time:
+    // the batch's live reads, then all of its model gathers, are issued before their use.
+    PassBatch pb = first;
+    const int p_first = blockIdx.x * 256 + tid;
+    for (int p0 = p_first; p0 < npx; p0 += kPxBatch * stride) {
+        if (p0 != p_first) loadBatch(a, p0, stride, pb);
+        int midx[kPxBatch];
 #pragma unroll
-        for (int b = 0; b < kPxBatch; ++b) {
-            const int p = p0 + b * stride;
-            idx[b] = p < npx ? (1 + p / iw) * a.W + 1 + p % iw : -1;
-        }
-#pragma unroll
-        for (int b = 0; b < kPxBatch; ++b) lv[b] = load3(a.lv, idx[b] < 0 ? 0 : idx[b]);
-#pragma unroll
-        for (int b = 0; b < kPxBatch; ++b) midx[b] = idx[b] < 0 ? -1 : icpProject(a, pd, lv[b], cnt);
-        F3 mv[kPxBatch], mn[kPxBatch], ln[kPxBatch];
+        for (int b = 0; b < kPxBatch; ++b) midx[b] = pb.idx[b] < 0 ? -1 : icpProject(a, pd, pb.lv[b], cnt);
+        F3 mv[kPxBatch], mn[kPxBatch];
 #pragma unroll
         for (int b = 0; b < kPxBatch; ++b) {
             const int m = midx[b] < 0 ? 0 : midx[b];
             mv[b] = load3(a.mv, m);
             mn[b] = load3(a.mn, m);
-            ln[b] = load3(a.ln, idx[b] < 0 ? 0 : idx[b]);
         }
 #pragma unroll
         for (int b = 0; b < kPxBatch; ++b)
-            if (midx[b] >= 0) icpAccumulate(a, pd, lv[b], mv[b], mn[b], ln[b], hi, lo, cnt);
+            if (midx[b] >= 0) icpAccumulate(a, pd, pb.lv[b], mv[b], mn[b], pb.ln[b], acc, cnt);
     }
 
-    // Block fold: every thread's pairs go to LDS; thread (s, g) folds slot s over the
+    // Block fold: every thread's sums go to LDS; thread (s, g) folds slot s over the
     // threads g, g + 9, g + 18, ... (compensated), and the nine group values are added
     // in FP64 in group order. Counters are exact integers (wave DPP sums).
 #pragma unroll
-    for (int i = 0; i < kIcpSlotsF; ++i) {
-        sh.red[2 * i][tid] = hi[i];
-        sh.red[2 * i + 1][tid] = lo[i];
-    }
+    for (int i = 0; i < kIcpSlotsF; ++i) sh.red[i][tid] = acc[i];
 #pragma unroll
     for (int i = 0; i < kIcpSlotsI; ++i) cnt[i] = waveSumInt(cnt[i]);
     if (lane == 63) {
@@ -684,7 +698,7 @@ __device__ int icpBlockPass(const IcpIterArgs& a, const PoseDerived& pd, double*
     if (tid < kIcpSlotsF * kFoldGroups) {
         const int sl = tid / kFoldGroups, g = tid % kFoldGroups;
         float h = 0.0f, l = 0.0f;
-        for (int j = g; j < 256; j += kFoldGroups) twoSum(h, l, sh.red[2 * sl][j], sh.red[2 * sl + 1][j]);
+        for (int j = g; j < 256; j += kFoldGroups) twoSum(h, l, sh.red[sl][j], 0.0f);
         sh.part[sl][g] = (double)h + (double)l;
     }
     __syncthreads();
@@ -697,33 +711,38 @@ __device__ int icpBlockPass(const IcpIterArgs& a, const PoseDerived& pd, double*
             const int i = tid - kIcpSlotsF;
             v = (double)(sh.cnt[0][i] + sh.cnt[1][i] + sh.cnt[2][i] + sh.cnt[3][i]);
         }
-        // Block-major (coalesced both ways). The stores come from wave 0, the wave whose
-        // thread 0 then releases them with its arrival.
+        // Block-major (coalesced both ways); the next launch reads them.
         partials[(size_t)blockIdx.x * kIcpSlots + tid] = v;
     }
-    return npx;
 }
 
-// The previous iteration's solve, in every block: FP64 cross-block sum of its partials,
-// then the solve. Thread (g, s) sums slot s over blocks g, g + 7, g + 14, ... with all of
-// its loads issued before the first add; the seven group sums are then added in group
-// order. The order is fixed, so every block gets the same sums, and the same solve.
-// Returns true when the level's loop ends; the new pose terms are left in sh.pd.
-__device__ bool blockSolve(IcpState* __restrict__ st, const double* __restrict__ partials, IcpShared& sh,
-                           const PoseDerived& pd, int level, int iter, int npx, int nblk, bool writer) {
+// Cross-block sum, first step: thread (g, s) reads slot s of blocks g, g + 7, g + 14, ...
+// of the previous launch's partials (zeros past its nblk blocks and for threads past
+// 7 * 34).
+constexpr int kMaxPer = (kIcpMaxBlocks + kSumGroups - 1) / kSumGroups;
+__device__ __forceinline__ void loadPartials(const double* __restrict__ partials, int nblk, double* v) {
+    const int tid = threadIdx.x;
+    const int s = tid % kIcpSlots, g = tid / kIcpSlots;
+    const unsigned nb = tid < kSumGroups * kIcpSlots ? (unsigned)nblk : 0u;
+    const double* src = partials + s;
+#pragma unroll
+    for (int i = 0; i < kMaxPer; ++i) {
+        const unsigned bi = (unsigned)(g + kSumGroups * i);
+        v[i] = bi < nb ? src[(size_t)bi * kIcpSlots] : 0.0;
+    }
+}
+
+// The previous iteration's solve, in every block: FP64 cross-block sum of its partials
+// (v: loadPartials), then the solve. Each thread adds its values in block order, and the
+// seven group sums are then added in group order. The order is fixed, so every block
+// gets the same sums, and the same solve. Returns true when the level's loop ends; the
+// new pose terms are left in sh.pd.
+__device__ bool blockSolve(IcpState* __restrict__ st, const double* v, IcpShared& sh, const PoseDerived& pd,
+                           int level, int iter, bool writer, unsigned long long* ticks) {
     const int tid = threadIdx.x;
     const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
-    const unsigned nb = (unsigned)nblk;
     if (tid < kSumGroups * kIcpSlots) {
         const int s = tid % kIcpSlots, g = tid / kIcpSlots;
-        const double* src = partials + s;
-        constexpr int kMaxPer = (kIcpMaxBlocks + kSumGroups - 1) / kSumGroups;
-        double v[kMaxPer];
-#pragma unroll
-        for (int i = 0; i < kMaxPer; ++i) {
-            const unsigned bi = (unsigned)(g + kSumGroups * i);
-            v[i] = bi < nb ? src[(size_t)bi * kIcpSlots] : 0.0;
-        }
         double acc = 0.0;
 #pragma unroll
         for (int i = 0; i < kMaxPer; ++i) acc += v[i];
@@ -744,17 +763,8 @@ __device__ bool blockSolve(IcpState* __restrict__ st, const double* __restrict__
             PoseDerived out;
             sh.stop = icpSolve(st, writer, sh.sum, &sh.lanes, pd, level, iter, out) ? 1 : 0;
             sh.pd = out;
-            if (writer) {
-                // diagnostics: atomics whose result is unused (no round trip)
-                const unsigned long long t_end = __builtin_amdgcn_s_memrealtime();
-                __hip_atomic_fetch_add(&st->ticks_sum, t_sum - t0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_fetch_add(&st->ticks_solve, t_end - t_sum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_fetch_add(&st->ticks_n, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_fetch_add(&st->work_px, (unsigned long long)npx, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_fetch_add(&st->work_proj, (unsigned long long)sh.sum[kIcpSlotsF + 3], __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-            }
+            ticks[0] = t_sum - t0;
+            ticks[1] = __builtin_amdgcn_s_memrealtime() - t_sum;
         }
     }
     __syncthreads();
@@ -781,18 +791,38 @@ __global__ __launch_bounds__(256) void icpIter(IcpIterArgs a, IcpPrev prev, IcpS
     const int stop = st->stop_level;
     const int pending = st->it[par].pending;
     PoseDerived pd = st->it[par].pd;
+    // Reads that depend on nothing this launch computes are issued before the state
+    // arrives: the previous launch's partials (used only if it ran a pass) and the first
+    // batch of this thread's live pixels, whose latency then hides under the solve.
+    double v[kMaxPer];
+    if (prev.level >= 0) loadPartials(part_prev, prev.nblk, v);
+    PassBatch first;
+    if (a.level >= 0) loadBatch(a, blockIdx.x * 256 + tid, (int)gridDim.x * 256, first);
     bool stopped = false;
-    if (pending && prev.level >= 0) {
-        stopped = blockSolve(st, part_prev, sh, pd, prev.level, prev.iter, prev.npx, prev.nblk, writer);
+    const bool solve = pending && prev.level >= 0;
+    unsigned long long ticks[2] = {0, 0};
+    if (solve) {
+        stopped = blockSolve(st, v, sh, pd, prev.level, prev.iter, writer, ticks);
         pd = sh.pd;
     }
     const bool pass = a.level >= 0 && stop != a.level && !(stopped && prev.level == a.level);
     if (writer && tid == 0) {
         st->it[1 - par].pd = pd;
         st->it[1 - par].pending = pass ? 1 : 0;
+        if (solve) {
+            // Diagnostics, after the block's last barrier of the solve: atomics whose
+            // result is unused, so nothing waits on them before the pass.
+            __hip_atomic_fetch_add(&st->ticks_sum, ticks[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&st->ticks_solve, ticks[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&st->ticks_n, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&st->work_px, (unsigned long long)prev.npx, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&st->work_proj, (unsigned long long)sh.sum[kIcpSlotsF + 3], __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        }
     }
     if (!pass) return;
-    icpBlockPass(a, pd, part_cur, sh, gridDim.x);
+    icpBlockPass(a, pd, part_cur, sh, first);
 }
 
 // ---------------------------------------------------------------------------
