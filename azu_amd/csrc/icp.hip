@@ -13,9 +13,9 @@
 //   2. then, unless the level has stopped, every thread walks its live pixels, associates
 //      each with the model image, builds J = [R^T n_m ; v x R^T n_m], the Huber weight
 //      and loss, and accumulates the 21 upper-triangle A terms, 6 b terms and the
-//      residual in FP32 with Kahan (TwoSum) compensation, plus 6 integer counters; the
-//      block folds its lanes through LDS, still as compensated pairs, and writes one
-//      FP64 partial per slot.
+//      residual in FP32 (fused multiply-adds over its few pixels), plus 6 integer
+//      counters; each wave sums its lanes by a fixed DPP tree, and the block adds its
+//      four wave sums in FP64 and writes one FP64 partial per slot.
 // The tracker state (pose, per-level result, stop flag) lives in device memory, so all
 // launches of a frame are enqueued back to back; once a level has stopped, its remaining
 // launches return at once.
@@ -60,6 +60,7 @@ struct IcpState {
         int pending;     // the launch before ran a pass: its partials await a solve
     } it[2];
     int stop_level;  // level whose iteration loop has ended, -1 while none
+    Rigid ref_inv;   // inverse of ref
 };
 
 struct IcpIterArgs {
@@ -75,20 +76,28 @@ struct IcpIterArgs {
     int level, iter;
 };
 
-// Compensated (TwoSum) accumulation of the pair (b_hi, b_lo) into (hi, lo).
-__device__ __forceinline__ void twoSum(float& hi, float& lo, float b_hi, float b_lo) {
-    const float s = hi + b_hi;
-    const float bp = s - hi;
-    const float e = (hi - (s - bp)) + (b_hi - bp);
-    hi = s;
-    lo = lo + b_lo + e;
-}
-
 // DPP lane move: lanes whose source is out of range or in a masked row read 0, the
 // identity of the sum below.
 template <int CTRL, int ROW_MASK = 0xf>
 __device__ __forceinline__ int dppI(int v) {
     return __builtin_amdgcn_update_dpp(0, v, CTRL, ROW_MASK, 0xf, true);
+}
+
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ float dppF(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xf, true));
+}
+
+// Wave64 float sum to lane 63, the same DPP sequence as waveSumInt below (a fixed
+// tree of six adds per lane).
+__device__ __forceinline__ float waveSumF(float v) {
+    v += dppF<0xb1, 0xf>(v);
+    v += dppF<0x4e, 0xf>(v);
+    v += dppF<0x114, 0xf>(v);
+    v += dppF<0x118, 0xf>(v);
+    v += dppF<0x142, 0xa>(v);
+    v += dppF<0x143, 0xc>(v);
+    return v;
 }
 
 // Wave64 integer sum to lane 63 with the gfx9 DPP sequence: quad_perm [1,0,3,2],
@@ -294,8 +303,11 @@ __device__ void projectSO3Svd(const double* M, double* R) {
 
 // The same projection for the matrices the update produces (a rotation times a
 // rotation, orthogonal up to float rounding): the polar factor by Newton's iteration
-// X <- (X + X^-T) / 2 (Higham, Functions of Matrices, 8.3), which converges
-// quadratically; it equals U V^T when det > 0. Anything else takes the SVD route.
+// X <- (X + X^-T) / 2 (Higham, Functions of Matrices, 8.3); it equals U V^T when
+// det > 0. Anything else takes the SVD route. With X = U (I + E), E symmetric, the step
+// leaves U (I + E^2 / 2 + O(E^3)) and moves X by about |E|: once a step moves no entry
+// by 1e-9, the iterate it produced is within FP64 rounding of U (E^2 / 2 < 1e-18).
+// From float-rounded input (|E| ~ 1e-7) that is two steps.
 __device__ void projectSO3(const double* M, double* R) {
     double X[9];
     for (int i = 0; i < 9; ++i) X[i] = M[i];
@@ -317,15 +329,15 @@ __device__ void projectSO3(const double* M, double* R) {
             dmax = fmax(dmax, fabs(nx - X[i]));
             X[i] = nx;
         }
-        if (dmax < 1e-15) break;
+        if (dmax < 1e-9) break;
     }
     for (int i = 0; i < 9; ++i) R[i] = X[i];
 }
 
 
-// The pose terms of a pose and reference pose (column-major 4 x 4 each).
-__device__ PoseDerived derivePose(const float* pose, const float* ref, int iterations) {
-    const Rigid P = rigidFromPtr(pose), rinv = rigidInverse(rigidFromPtr(ref));
+// The pose terms of a pose (column-major 4 x 4) and the inverse of the reference pose.
+__device__ PoseDerived derivePose(const float* pose, const Rigid& rinv, int iterations) {
+    const Rigid P = rigidFromPtr(pose);
     PoseDerived d;
     d.R_cw = P.R;
     d.t_cw = P.t;
@@ -338,10 +350,11 @@ __device__ PoseDerived derivePose(const float* pose, const float* ref, int itera
 // One solve + pose update (trackLevel's loop body, ICPTracker.cpp:79-130), lane 0 of
 // wave 0 once solveLane has run on lanes 0-2; every block of a launch runs it on the same
 // sums and gets the same result, and the writer (block 0) also records it in the state.
+// It reads no global memory (ref_inv: the state's, read at the launch's start).
 // sums: 28 FP64 slots (21 A upper, 6 b, residual) + 6 counters. out: the pose terms after
 // the step (pd when there is none); returns true when the level's loop ends.
 __device__ bool icpSolve(IcpState* st, bool writer, const double* sums, const SolveLanes* sl, const PoseDerived& pd,
-                         int level, int iter, PoseDerived& out) {
+                         const Rigid& ref_inv, int level, int iter, PoseDerived& out) {
     out = pd;
     if (writer && iter == 0) {
         st->error = 0.0f;
@@ -434,18 +447,16 @@ __device__ bool icpSolve(IcpState* st, bool writer, const double* sums, const So
     double Rp[9];
     projectSO3(Md, Rp);
     float np[16] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1};
-    if (writer)
-        for (int i = 0; i < 16; ++i) np[i] = st->pose[i];  // the bottom row stays as it was
     for (int i = 0; i < 3; ++i) {
         for (int j = 0; j < 3; ++j) np[j * 4 + i] = (float)Rp[i * 3 + j];
         np[12 + i] = nt[i];
     }
-    float rf[16];
-    for (int i = 0; i < 16; ++i) rf[i] = st->ref[i];  // constant over the track
-    out = derivePose(np, rf, pd.iterations + (1 << (8 * level)));
+    out = derivePose(np, ref_inv, pd.iterations + (1 << (8 * level)));
     const bool converged = step_norm < 5e-5f;
     if (writer) {
-        for (int i = 0; i < 16; ++i) st->pose[i] = np[i];
+        // Rows 0-2; the bottom row stays as it was.
+        for (int j = 0; j < 4; ++j)
+            for (int i = 0; i < 3; ++i) st->pose[j * 4 + i] = np[j * 4 + i];
         st->final_step = step_norm;
         st->last_final_step = step_norm;
         st->iterations = pd.iterations + (1 << (8 * level));
@@ -480,8 +491,10 @@ __global__ void icpInit(IcpState* st, PoseArg est, PoseArg ref) {
     for (int i = 0; i < 5; ++i) st->counters[i] = 0;
     st->stop_level = -1;
     st->iterations = 0;
+    const Rigid rinv = rigidInverse(rigidFromPtr(r));
+    st->ref_inv = rinv;
     for (int p = 0; p < 2; ++p) {
-        st->it[p].pd = derivePose(e, r, 0);
+        st->it[p].pd = derivePose(e, rinv, 0);
         st->it[p].pending = 0;
     }
 }
@@ -579,9 +592,9 @@ __device__ __forceinline__ void icpAccumulate(const IcpIterArgs& a, const PoseDe
     const float w = (ae <= 0.02f) ? 1.0f : 0.02f / ae;
     const float we = err * w;
     const float loss = (ae <= 0.02f) ? ae * ae : 2.0f * 0.02f * ae - 0.02f * 0.02f;
-    // A thread folds a handful of pixels (at most kPxBatch per level-0 pass), so plain
-    // fused accumulation keeps its sums within a few float roundings; the long sums run
-    // compensated in the block fold and in FP64 across blocks.
+    // A thread folds a handful of pixels (at most kPxBatch per level-0 pass of a
+    // 640 x 480 frame), and its wave 64 such sums, so the float sums stay within a few
+    // roundings of the terms' magnitudes; everything longer runs in FP64.
     int k = 0;
 #pragma unroll
     for (int i = 0; i < 6; ++i) {
@@ -601,7 +614,6 @@ __device__ __forceinline__ void icpAccumulate(const IcpIterArgs& a, const PoseDe
 __device__ __forceinline__ F3 load3(const float* __restrict__ p, int i) { return F3{p[3 * i], p[3 * i + 1], p[3 * i + 2]}; }
 
 constexpr int kPxBatch = 5;   // pixels whose reads one thread issues together
-constexpr int kFoldGroups = 9;  // block fold: threads per system slot (28 * 9 = 252 <= 256)
 
 // Blocks an iteration over a W x H level is split into: one per 256 interior pixels,
 // at most kIcpMaxBlocks.
@@ -637,8 +649,7 @@ __device__ __forceinline__ void loadBatch(const IcpIterArgs& a, int p0, int stri
 }
 
 struct IcpShared {
-    float red[kIcpSlotsF][256];  // every thread's sums, slot-major
-    double part[kIcpSlotsF][kFoldGroups];
+    float wsum[4][kIcpSlotsF];  // per-wave sums
     int cnt[4][kIcpSlotsI];
     double grp[kSumGroups][kIcpSlots];
     double sum[kIcpSlots];
@@ -683,30 +694,24 @@ __device__ void icpBlockPass(const IcpIterArgs& a, const PoseDerived& pd, double
             if (midx[b] >= 0) icpAccumulate(a, pd, pb.lv[b], mv[b], mn[b], pb.ln[b], acc, cnt);
     }
 
-    // Block fold: every thread's sums go to LDS; thread (s, g) folds slot s over the
-    // threads g, g + 9, g + 18, ... (compensated), and the nine group values are added
-    // in FP64 in group order. Counters are exact integers (wave DPP sums).
+    // Block fold: a fixed DPP tree per wave in float (64 thread sums of a few terms
+    // each), then the four wave sums in FP64 in wave order. Counters are exact integers.
 #pragma unroll
-    for (int i = 0; i < kIcpSlotsF; ++i) sh.red[i][tid] = acc[i];
+    for (int i = 0; i < kIcpSlotsF; ++i) acc[i] = waveSumF(acc[i]);
 #pragma unroll
     for (int i = 0; i < kIcpSlotsI; ++i) cnt[i] = waveSumInt(cnt[i]);
     if (lane == 63) {
 #pragma unroll
+        for (int i = 0; i < kIcpSlotsF; ++i) sh.wsum[wv][i] = acc[i];
+#pragma unroll
         for (int i = 0; i < kIcpSlotsI; ++i) sh.cnt[wv][i] = cnt[i];
-    }
-    __syncthreads();
-    if (tid < kIcpSlotsF * kFoldGroups) {
-        const int sl = tid / kFoldGroups, g = tid % kFoldGroups;
-        float h = 0.0f, l = 0.0f;
-        for (int j = g; j < 256; j += kFoldGroups) twoSum(h, l, sh.red[sl][j], 0.0f);
-        sh.part[sl][g] = (double)h + (double)l;
     }
     __syncthreads();
     if (tid < kIcpSlots) {
         double v;
         if (tid < kIcpSlotsF) {
-            v = 0.0;
-            for (int g = 0; g < kFoldGroups; ++g) v += sh.part[tid][g];
+            v = (((double)sh.wsum[0][tid] + (double)sh.wsum[1][tid]) + (double)sh.wsum[2][tid]) +
+                (double)sh.wsum[3][tid];
         } else {
             const int i = tid - kIcpSlotsF;
             v = (double)(sh.cnt[0][i] + sh.cnt[1][i] + sh.cnt[2][i] + sh.cnt[3][i]);
@@ -738,7 +743,7 @@ __device__ __forceinline__ void loadPartials(const double* __restrict__ partials
 // gets the same sums, and the same solve. Returns true when the level's loop ends; the
 // new pose terms are left in sh.pd.
 __device__ bool blockSolve(IcpState* __restrict__ st, const double* v, IcpShared& sh, const PoseDerived& pd,
-                           int level, int iter, bool writer, unsigned long long* ticks) {
+                           const Rigid& ref_inv, int level, int iter, bool writer, unsigned long long* ticks) {
     const int tid = threadIdx.x;
     const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
     if (tid < kSumGroups * kIcpSlots) {
@@ -761,7 +766,7 @@ __device__ bool blockSolve(IcpState* __restrict__ st, const double* v, IcpShared
         waveSync();
         if (tid == 0) {
             PoseDerived out;
-            sh.stop = icpSolve(st, writer, sh.sum, &sh.lanes, pd, level, iter, out) ? 1 : 0;
+            sh.stop = icpSolve(st, writer, sh.sum, &sh.lanes, pd, ref_inv, level, iter, out) ? 1 : 0;
             sh.pd = out;
             ticks[0] = t_sum - t0;
             ticks[1] = __builtin_amdgcn_s_memrealtime() - t_sum;
@@ -791,6 +796,7 @@ __global__ __launch_bounds__(256) void icpIter(IcpIterArgs a, IcpPrev prev, IcpS
     const int stop = st->stop_level;
     const int pending = st->it[par].pending;
     PoseDerived pd = st->it[par].pd;
+    const Rigid ref_inv = st->ref_inv;
     // Reads that depend on nothing this launch computes are issued before the state
     // arrives: the previous launch's partials (used only if it ran a pass) and the first
     // batch of this thread's live pixels, whose latency then hides under the solve.
@@ -802,7 +808,7 @@ __global__ __launch_bounds__(256) void icpIter(IcpIterArgs a, IcpPrev prev, IcpS
     const bool solve = pending && prev.level >= 0;
     unsigned long long ticks[2] = {0, 0};
     if (solve) {
-        stopped = blockSolve(st, v, sh, pd, prev.level, prev.iter, writer, ticks);
+        stopped = blockSolve(st, v, sh, pd, ref_inv, prev.level, prev.iter, writer, ticks);
         pd = sh.pd;
     }
     const bool pass = a.level >= 0 && stop != a.level && !(stopped && prev.level == a.level);

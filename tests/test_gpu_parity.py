@@ -2,8 +2,8 @@
 
 Bars: integration, raycast and the live pyramid are bit-identical (the exact kernels
 run the oracle's float operation sequence); the ICP pose agrees within 1e-5 m /
-1e-5 rad (FP32 terms with compensated FP64 cross-block sums on the GPU against the
-reference's float accumulation in the oracle).
+1e-5 rad (FP32 terms and per-wave FP32 sums, FP64 across waves and blocks, on the
+GPU against the reference's float accumulation in the oracle).
 """
 import numpy as np
 import pytest
