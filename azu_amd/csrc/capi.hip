@@ -747,7 +747,8 @@ int azu_tracker_create(int W, int H, const float K[4], const azu_icp_params* p, 
         azu_tracker_destroy(t);
         return fail(AZU_ERR_ALLOC, "tracker allocation failed");
     }
-    if (hipMemset(t->state, 0, icpStateBytes()) != hipSuccess) {
+    if (hipMemset(t->state, 0, icpStateBytes()) != hipSuccess ||
+        hipMemset(t->partials, 0, icpPartialsBytes(1024)) != hipSuccess) {
         azu_tracker_destroy(t);
         return fail(AZU_ERR_HIP, "tracker initialisation failed");
     }

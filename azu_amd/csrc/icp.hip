@@ -722,10 +722,12 @@ __device__ void icpBlockPass(const IcpIterArgs& a, const PoseDerived& pd, double
 }
 
 // Cross-block sum, first step: thread (g, s) reads slot s of blocks g, g + 7, g + 14, ...
-// of the previous launch's partials (zeros past its nblk blocks and for threads past
-// 7 * 34).
+// of the previous launch's partials (threads past 7 * 34: zeros). Reads
+// past the previous launch's nblk blocks go to the zero row kIcpMaxBlocks, which no
+// launch writes, so every read is unconditional and all are in flight at once.
 constexpr int kMaxPer = (kIcpMaxBlocks + kSumGroups - 1) / kSumGroups;
-__device__ __forceinline__ void loadPartials(const double* __restrict__ partials, int nblk, double* v) {
+constexpr int kPartialRows = kIcpMaxBlocks + 1;
+__device__ __forceinline__ void loadPartials(const double* __restrict__ partials, int nblk, double* x) {
     const int tid = threadIdx.x;
     const int s = tid % kIcpSlots, g = tid / kIcpSlots;
     const unsigned nb = tid < kSumGroups * kIcpSlots ? (unsigned)nblk : 0u;
@@ -733,26 +735,20 @@ __device__ __forceinline__ void loadPartials(const double* __restrict__ partials
 #pragma unroll
     for (int i = 0; i < kMaxPer; ++i) {
         const unsigned bi = (unsigned)(g + kSumGroups * i);
-        v[i] = bi < nb ? src[(size_t)bi * kIcpSlots] : 0.0;
+        x[i] = src[(size_t)(bi < nb ? bi : (unsigned)kIcpMaxBlocks) * kIcpSlots];
     }
 }
 
 // The previous iteration's solve, in every block: FP64 cross-block sum of its partials
-// (v: loadPartials), then the solve. Each thread adds its values in block order, and the
-// seven group sums are then added in group order. The order is fixed, so every block
+// (gsum: the thread's loadPartials values added in block order), then the solve. The seven group sums are added in
+// group order. The order is fixed, so every block
 // gets the same sums, and the same solve. Returns true when the level's loop ends; the
 // new pose terms are left in sh.pd.
-__device__ bool blockSolve(IcpState* __restrict__ st, const double* v, IcpShared& sh, const PoseDerived& pd,
+__device__ bool blockSolve(IcpState* __restrict__ st, double gsum, IcpShared& sh, const PoseDerived& pd,
                            const Rigid& ref_inv, int level, int iter, bool writer, unsigned long long* ticks) {
     const int tid = threadIdx.x;
     const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
-    if (tid < kSumGroups * kIcpSlots) {
-        const int s = tid % kIcpSlots, g = tid / kIcpSlots;
-        double acc = 0.0;
-#pragma unroll
-        for (int i = 0; i < kMaxPer; ++i) acc += v[i];
-        sh.grp[g][s] = acc;
-    }
+    if (tid < kSumGroups * kIcpSlots) sh.grp[tid / kIcpSlots][tid % kIcpSlots] = gsum;
     __syncthreads();
     if (tid < kIcpSlots) {
         double acc = 0.0;
@@ -790,26 +786,53 @@ __global__ __launch_bounds__(256) void icpIter(IcpIterArgs a, IcpPrev prev, IcpS
     __shared__ IcpShared sh;
     const int tid = threadIdx.x;
     const bool writer = blockIdx.x == 0;
+    // Every kernel argument is in registers before the first state read: one scalar
+    // wait covers all scalar loads in flight, so an argument first loaded after the
+    // state read was issued would wait out the state's round trip too. The empty asm
+    // statements pass the level through as if they depended on every argument, and the
+    // level decides the first branch.
+    int level = a.level;
+    asm("" : "+s"(level) : "s"(a.lv), "s"(a.ln), "s"(a.mv), "s"(a.mn), "s"(a.W), "s"(a.H), "s"(a.MW), "s"(a.MH));
+    asm("" : "+s"(level) : "s"(a.Km.fx), "s"(a.Km.fy), "s"(a.Km.cx), "s"(a.Km.cy), "s"(a.angle_cos), "s"(a.dist_sq),
+        "s"(a.iter), "s"((int)gridDim.x));
+    asm("" : "+s"(level) : "s"(prev.level), "s"(prev.iter), "s"(prev.npx), "s"(prev.nblk), "s"(st), "s"(part_prev),
+        "s"(part_cur), "s"(par));
     // Read at the start: the stop level (block 0 of this launch may set it, to a value
     // every block also reaches by its own solve) and this launch's slot (written by the
     // launch before; block 0 writes the other one).
+    // The first launch of a level always runs its pass (a level stops only in its own
+    // launches); its first batch of live pixels is requested before anything else.
+    // Later launches of a level request it once the state shows the level still running
+    // (otherwise the launch has nothing to do), in time to arrive under the solve.
+    const bool level_start = level >= 0 && level != prev.level;
+    const int p_first = blockIdx.x * 256 + tid, stride = (int)gridDim.x * 256;
+    PassBatch first;
+    if (level_start) loadBatch(a, p_first, stride, first);
     const int stop = st->stop_level;
     const int pending = st->it[par].pending;
     PoseDerived pd = st->it[par].pd;
     const Rigid ref_inv = st->ref_inv;
-    // Reads that depend on nothing this launch computes are issued before the state
-    // arrives: the previous launch's partials (used only if it ran a pass) and the first
-    // batch of this thread's live pixels, whose latency then hides under the solve.
-    double v[kMaxPer];
-    if (prev.level >= 0) loadPartials(part_prev, prev.nblk, v);
-    PassBatch first;
-    if (a.level >= 0) loadBatch(a, blockIdx.x * 256 + tid, (int)gridDim.x * 256, first);
+    // The previous launch's partials (used only if it ran a pass) are requested before
+    // the state arrives.
+    double x[kMaxPer];
+    if (prev.level >= 0) loadPartials(part_prev, prev.nblk, x);
+    const bool late = !level_start && a.level >= 0 && stop != a.level;
     bool stopped = false;
     const bool solve = pending && prev.level >= 0;
     unsigned long long ticks[2] = {0, 0};
     if (solve) {
-        stopped = blockSolve(st, v, sh, pd, ref_inv, prev.level, prev.iter, writer, ticks);
+        // The thread's share of the sum is taken before the late request of the live
+        // pixels, so that waiting for the partials does not wait for those too (the
+        // empty asm keeps the sum here).
+        double gsum = x[0];
+#pragma unroll
+        for (int i = 1; i < kMaxPer; ++i) gsum += x[i];
+        asm volatile("" ::"v"(gsum));
+        if (late) loadBatch(a, p_first, stride, first);
+        stopped = blockSolve(st, gsum, sh, pd, ref_inv, prev.level, prev.iter, writer, ticks);
         pd = sh.pd;
+    } else if (late) {
+        loadBatch(a, p_first, stride, first);
     }
     const bool pass = a.level >= 0 && stop != a.level && !(stopped && prev.level == a.level);
     if (writer && tid == 0) {
@@ -836,8 +859,9 @@ __global__ __launch_bounds__(256) void icpIter(IcpIterArgs a, IcpPrev prev, IcpS
 // ---------------------------------------------------------------------------
 size_t icpStateBytes() { return sizeof(IcpState); }
 
-// Two halves (the launch's own and the previous launch's partials).
-size_t icpPartialsBytes(int) { return 2 * sizeof(double) * kIcpSlots * (size_t)kIcpMaxBlocks; }
+// Two halves (the launch's own and the previous launch's partials), each with a zero row
+// after its kIcpMaxBlocks block rows; the buffer is zeroed once at allocation.
+size_t icpPartialsBytes(int) { return 2 * sizeof(double) * kIcpSlots * (size_t)kPartialRows; }
 
 hipError_t launchIcpInit(void* st, const PoseArg& est, const PoseArg& ref, hipStream_t s) {
     hipLaunchKernelGGL(icpInit, dim3(1), dim3(64), 0, s, (IcpState*)st, est, ref);
@@ -873,7 +897,7 @@ hipError_t launchIcpIter(const float* lv, const float* ln, int W, int H, const f
         prev.npx = (prev_W > 2 && prev_H > 2) ? (prev_W - 2) * (prev_H - 2) : 0;
         prev.nblk = icpBlocksFor(prev_W, prev_H);
     }
-    const size_t half = (size_t)kIcpSlots * kIcpMaxBlocks;
+    const size_t half = (size_t)kIcpSlots * kPartialRows;
     double* cur = partials + (size_t)(k & 1) * half;
     const double* old = partials + (size_t)((k + 1) & 1) * half;
     const unsigned blocks = level >= 0 ? (unsigned)icpBlocksFor(W, H) : 1u;
