@@ -27,7 +27,7 @@ hipError_t launchBrickMap(int format, const void* vox, int res, unsigned char* b
 hipError_t launchRaycast(int format, bool exact, const void* vox, const VolParams& p, const FixedDiv& dvs,
                          const unsigned char* bricks, unsigned* dsafe, int* ok, const PoseArg& pa, Intr K, int W,
                          int H, float* verts, float* norms, float* t_hit, hipStream_t st);
-hipError_t launchSkipInit(unsigned* dsafe, size_t n, hipStream_t st);
+hipError_t launchSkipInit(unsigned* dsafe, size_t cells, hipStream_t st);
 hipError_t launchLive(bool mm, const void* src, int W, int H, Intr K, float min_d, float max_d, float* const d[3],
                       float* const v[3], float* const n[3], hipStream_t st);
 size_t icpStateBytes();
@@ -215,8 +215,8 @@ struct azu_volume {
     unsigned char* bricks = nullptr;
     bool external = false;
     bool skip = true;  // empty-space skipping in the raycast
-    unsigned* dsafe = nullptr;  // per-pixel skip bounds (+inf between raycasts), then one flag word
-    size_t dsafe_px = 0;
+    unsigned* dsafe = nullptr;  // skip bounds, two words per 2 x 2 pixel cell, then one flag word
+    size_t dsafe_cells = 0;
     bool exact() const { return mode != AZU_MODE_FAST; }
 };
 
@@ -577,24 +577,24 @@ static int raycastImpl(azu_volume* v, const PoseArg& pa, const float* K, int W, 
     ProfScope ps(exact ? (v->format == AZU_FMT_F32 ? "raycast_exact_f32" : "raycast_exact_f16")
                        : (v->format == AZU_FMT_F32 ? "raycast_fast_f32" : "raycast_fast_f16"),
                  (hipStream_t)s);
-    const size_t px = (size_t)W * H;
-    if (px > v->dsafe_px) {
+    const size_t cells = (size_t)((W + 1) / 2) * (size_t)((H + 1) / 2);
+    if (cells > v->dsafe_cells) {
         // grows only with the image; work queued on other streams may still use the old one
         AZU_HIP_TRY(hipDeviceSynchronize(), "synchronise before skip scratch reallocation");
         if (v->dsafe) (void)hipFree(v->dsafe);
         v->dsafe = nullptr;
-        v->dsafe_px = 0;
-        if (hipMalloc(&v->dsafe, (px + 1) * sizeof(unsigned)) != hipSuccess)
+        v->dsafe_cells = 0;
+        if (hipMalloc(&v->dsafe, (2 * cells + 1) * sizeof(unsigned)) != hipSuccess)
             return fail(AZU_ERR_ALLOC, "raycast skip scratch allocation failed");
-        AZU_HIP_TRY(launchSkipInit(v->dsafe, px, (hipStream_t)s), "skip scratch init");
-        v->dsafe_px = px;
+        AZU_HIP_TRY(launchSkipInit(v->dsafe, cells, (hipStream_t)s), "skip scratch init");
+        v->dsafe_cells = cells;
     }
     if (v->skip && v->external)
         AZU_HIP_TRY(launchBrickMap(v->format, v->vox, v->params.resolution, v->bricks, true, (hipStream_t)s),
                     "brick map rebuild");
     AZU_HIP_TRY(launchRaycast(v->format, exact, v->vox, toVol(v->params), v->dvs, v->skip ? v->bricks : nullptr,
                               v->dsafe,
-                              reinterpret_cast<int*>(v->dsafe + v->dsafe_px), pa, Intr{K[0], K[1], K[2], K[3]}, W,
+                              reinterpret_cast<int*>(v->dsafe + 2 * v->dsafe_cells), pa, Intr{K[0], K[1], K[2], K[3]}, W,
                               H, dv, dn, dt, (hipStream_t)s),
                 "raycast launch");
     return AZU_OK;

@@ -78,7 +78,10 @@ struct VolBuf {
 // span (or the cell is outside the volume), so the sample is far, and the march takes
 // its far step there without reading it.
 struct RaySkip {
-    unsigned* dsafe;  // per 2 x 2 pixel cell, float bit patterns (+inf between calls), or null
+    // Per 2 x 2 pixel cell, two float bit patterns: the least distance from the camera of
+    // any marked brick the cell's rays can meet (+inf between calls) and the greatest (0
+    // between calls); or null.
+    unsigned* dsafe;
     const int* ok;
 };
 
@@ -251,12 +254,21 @@ __global__ __launch_bounds__(256) void raycastKernel(Vol<S> V, VolParams p, Pose
         R.sy = (unsigned)V.res * (unsigned)S::kBytes;
         R.sz = R.sy * (unsigned)V.res;
     }
-    // XCD-aware tile order: XCD k (blocks b with b % 8 == k) gets a contiguous band.
+    // Tile order. FP32 words: launch order, so that consecutive tiles run on different
+    // XCDs (blocks are dealt to the XCDs in turn): the long rays cluster in a few image
+    // regions, and a contiguous band per XCD leaves their memory traffic to one XCD's L2
+    // (measured 338 -> 297 us per launch at 512^3). FP16 words: the band per XCD (block b
+    // on XCD b % 8), which measures a few microseconds faster there.
     const int ntx = (W + 15) / 16, nty = (H + 15) / 16, nt = ntx * nty;
-    const int b = blockIdx.x, xcd = b & 7, k = b >> 3;
-    const int per = nt >> 3, rem = nt & 7;
-    if (k >= per + (xcd < rem ? 1 : 0)) return;
-    const int tile = xcd * per + min(xcd, rem) + k;
+    int tile = blockIdx.x;
+    if (S::kBytes == 8) {
+        if (tile >= nt) return;
+    } else {
+        const int b = blockIdx.x, xcd = b & 7, k = b >> 3;
+        const int per = nt >> 3, rem = nt & 7;
+        if (k >= per + (xcd < rem ? 1 : 0)) return;
+        tile = xcd * per + min(xcd, rem) + k;
+    }
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int x = (tile % ntx) * 16 + (wv & 1) * 8 + (lane & 7);
     const int y = (tile / ntx) * 16 + (wv >> 1) * 8 + (lane >> 3);
@@ -300,21 +312,27 @@ __global__ __launch_bounds__(256) void raycastKernel(Vol<S> V, VolParams p, Pose
     // camera so far away that the float spacing of t exceeds twice the step) the march
     // would never end, and the ray is left a miss.
     if (!(fabsf(t_far) + min_step > fabsf(t_far)) || !(fabsf(t_near) + min_step > fabsf(t_near))) t_far = -1.0f;
-    // The skip bound of this pixel's 2 x 2 cell (the cell lies in this wave's 8 x 8 tile;
-    // its first pixel resets it to +inf at the end, after every lane has read it).
+    // The skip bounds of this pixel's 2 x 2 cell (the cell lies in this wave's 8 x 8 tile;
+    // its first pixel resets them at the end, after every lane has read them). Samples
+    // before t_safe and after t_end are far.
     const int cell = (y >> 1) * ((W + 1) >> 1) + (x >> 1);
-    float t_safe = -__builtin_inff();
+    float t_safe = -__builtin_inff(), t_end = __builtin_inff();
     if (skip.dsafe) {
-        const float d = __uint_as_float(skip.dsafe[cell]);
-        if (*skip.ok) t_safe = d / f3norm(dir);
+        const uint2 d = *reinterpret_cast<const uint2*>(skip.dsafe + 2 * (size_t)cell);
+        if (*skip.ok) {
+            const float dn = f3norm(dir);
+            t_safe = __uint_as_float(d.x) / dn;
+            t_end = __uint_as_float(d.y) / dn;
+        }
     }
     bool found = false;
     float t_hit = 0.0f;
     F3 hit{0, 0, 0}, nrm{0, 0, 0};
     if (t_near < t_far) {
         float t_prev = t_near, f_prev = 1.0f;
-        bool prev_valid = false, go = true;
-        if (t_near < t_safe) {
+        // Every sample lies at t >= t_near; past t_end all are far, so none can hit.
+        bool prev_valid = false, go = t_near < t_end;
+        if (go && t_near < t_safe) {
             // Samples before t_safe are far: advance by the far-step recurrence without
             // taking them, then take the last one skipped (its value enters the next
             // step's crossing test).
@@ -332,7 +350,8 @@ __global__ __launch_bounds__(256) void raycastKernel(Vol<S> V, VolParams p, Pose
             prev_valid = sampleVol<S, E, FD, B>(V, R, f3add(cam, f3mul(dir, t_prev)), f_prev);
             go = !(prev_valid && (!isfinite(f_prev) || f_prev < 0.0f));
         }
-        while (go && t_prev < t_far) {
+        // Once t_prev >= t_end every later sample is far: no hit can follow.
+        while (go && t_prev < t_far && t_prev < t_end) {
             if (kSpecOf<S> > 0 && (!prev_valid || f_prev >= 0.999f)) {
                 // A run of far steps: while every sample is unobserved or >= 0.999 the
                 // step stays far_step, so the next kSpec parameters follow from t_prev
@@ -424,7 +443,7 @@ __global__ __launch_bounds__(256) void raycastKernel(Vol<S> V, VolParams p, Pose
     no[1] = nrm.y;
     no[2] = nrm.z;
     if (out.t_hit) out.t_hit[o] = found ? t_hit : __builtin_nanf("");
-    if (skip.dsafe && ((x | y) & 1) == 0) skip.dsafe[cell] = 0x7f800000u;
+    if (skip.dsafe && ((x | y) & 1) == 0) *reinterpret_cast<uint2*>(skip.dsafe + 2 * (size_t)cell) = make_uint2(0x7f800000u, 0u);
 }
 
 template <class S>
@@ -445,7 +464,7 @@ template <class S, bool E, int FD>
 void launchRay(const void* vox, const VolParams& p, const FixedDiv& dvs, const RaySkip& skip, const PoseArg& pa,
                Intr K, int W, int H, const RayOut& out, hipStream_t st) {
     const int nt = ((W + 15) / 16) * ((H + 15) / 16);
-    const unsigned blocks = (unsigned)(((nt + 7) / 8) * 8);
+    const unsigned blocks = (unsigned)(S::kBytes == 8 ? nt : ((nt + 7) / 8) * 8);
     // The buffer form measures faster for the FP32 words (344 against 412 us per launch
     // at 512^3) and slightly slower for the FP16 ones (224 against 218 us).
     const bool small = S::kBytes == 8 && (size_t)p.res * p.res * p.res * S::kBytes <= ((size_t)1 << 31);
@@ -467,8 +486,12 @@ void launchRay(const void* vox, const VolParams& p, const FixedDiv& dvs, const R
 // hold a sample (every sample lies at camera depth >= min_depth), so the corners there
 // and the box edges' crossings of the plane are projected; the perspective image of that
 // convex part is their hull, and the pixel rectangle around it, widened by one pixel,
-// takes the minimum of the box's distance from the camera (less 1e-5 of it) in every
-// 2 x 2 pixel cell it touches, one cell per lane. A sample at parameter t is the point cam + dir t, at distance t |dir|.
+// takes the minimum of the box's distance from the camera (less 1e-5 of it) and the
+// maximum of its farthest point's distance (plus 1e-5 of it) in every 2 x 2 pixel cell
+// it touches, one cell per lane. A sample at parameter t is the point cam + dir t, at
+// distance t |dir|; a sample that is not far lies in some marked brick's widened box, in
+// front of the plane, so its pixel is in that brick's rectangle and its distance between
+// the two bounds.
 // ok = 0 (take every sample) unless the pose is finite and invertible and min_depth > 0.
 __global__ __launch_bounds__(256) void brickSplat(const unsigned char* __restrict__ bricks, int nb, VolParams p,
                                                   PoseArg pa, Intr K, int W, int H, unsigned* __restrict__ dsafe,
@@ -508,15 +531,18 @@ __global__ __launch_bounds__(256) void brickSplat(const unsigned char* __restric
                      2.0f * (fabsf(p.origin[0]) + fabsf(p.origin[1]) + fabsf(p.origin[2]))) / p.vs +
                     12.0f * (float)p.res + 32.0f;
     const float e = 0.01f + M * 0x1p-17f;
-    float lo[3], hi[3], d2 = 0.0f;
+    float lo[3], hi[3], d2 = 0.0f, g2 = 0.0f;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         lo[a] = p.origin[a] + p.vs * ((float)(kBrick * b[a]) - e);
         hi[a] = p.origin[a] + p.vs * ((float)(kBrick * b[a] + kBrick) + e);
         const float d = fmaxf(fmaxf(lo[a] - cam_a[a], cam_a[a] - hi[a]), 0.0f);
+        const float g = fmaxf(cam_a[a] - lo[a], hi[a] - cam_a[a]);
         d2 += d * d;
+        g2 += g * g;
     }
     const unsigned dlow = __float_as_uint(sqrtf(d2) * (1.0f - 1e-5f));
+    const unsigned dhigh = __float_as_uint(sqrtf(g2) * (1.0f + 1e-5f));
     F3 q[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k)
@@ -555,24 +581,29 @@ __global__ __launch_bounds__(256) void brickSplat(const unsigned char* __restric
     if (x0 > x1 || y0 > y1) return;
     const int cw = (W + 1) >> 1, cx0 = x0 >> 1, cy0 = y0 >> 1;
     const int w = (x1 >> 1) - cx0 + 1, area = w * ((y1 >> 1) - cy0 + 1);
-    for (int k = lane; k < area; k += 64) atomicMin(&dsafe[(size_t)(cy0 + k / w) * cw + (cx0 + k % w)], dlow);
+    for (int k = lane; k < area; k += 64) {
+        unsigned* c = dsafe + 2 * ((size_t)(cy0 + k / w) * cw + (cx0 + k % w));
+        atomicMin(c, dlow);
+        atomicMax(c + 1, dhigh);
+    }
 }
 
-__global__ void fillWords(unsigned* __restrict__ p, size_t n, unsigned v) {
+__global__ void fillPairs(uint2* __restrict__ p, size_t n, uint2 v) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         p[i] = v;
 }
 
-// The skip bounds' scratch in its resting state (+inf everywhere).
-hipError_t launchSkipInit(unsigned* dsafe, size_t n, hipStream_t st) {
-    hipLaunchKernelGGL(fillWords, dim3((unsigned)std::min<size_t>((n + 255) / 256, 4096)), dim3(256), 0, st, dsafe,
-                       n, 0x7f800000u);
+// The skip bounds' scratch in its resting state: cells cells of (+inf, 0).
+hipError_t launchSkipInit(unsigned* dsafe, size_t cells, hipStream_t st) {
+    hipLaunchKernelGGL(fillPairs, dim3((unsigned)std::min<size_t>((cells + 255) / 256, 4096)), dim3(256), 0, st,
+                       reinterpret_cast<uint2*>(dsafe), cells, make_uint2(0x7f800000u, 0u));
     return hipGetLastError();
 }
 
-// bricks: the volume's brick map, or null to take every sample; dsafe (>= one word per
-// 2 x 2 pixel cell, all
-// +inf on entry and on return) and ok (one int): the skip bounds' scratch.
+// bricks: the volume's brick map, or null to take every sample; dsafe (two words per
+// 2 x 2 pixel cell, ((W + 1) / 2) ((H + 1) / 2) cells, 8-byte aligned, in the resting
+// state of launchSkipInit on entry and on return) and ok (one int): the skip bounds'
+// scratch.
 hipError_t launchRaycast(int format, bool exact, const void* vox, const VolParams& p, const FixedDiv& dvs,
                          const unsigned char* bricks, unsigned* dsafe, int* ok, const PoseArg& pa, Intr K, int W,
                          int H, float* verts, float* norms, float* t_hit, hipStream_t st) {
