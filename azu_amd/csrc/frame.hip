@@ -5,10 +5,10 @@
 //    vertex, and the 4-neighbour cross-product normal with the depth-jump guard. The
 //    neighbours' vertices are recomputed from their depths (same formula, same bits),
 //    so one kernel produces the whole level.
-//  * livePyramid: per output pixel of level 2, the four level-1 pixels of its 2x2
-//    block from the 4x4 level-0 block beneath (2x2 mean of valid samples, or the first
-//    nearest sample when the block straddles a depth jump), then the level-2 pixel from
-//    those four. Levels 1 and 2 keep the averaged, renormalised normals like the CPU path.
+//  * livePyramid: per level-1 pixel, the 2x2 level-0 block beneath (2x2 mean of valid
+//    samples, or the first nearest sample when the block straddles a depth jump); the
+//    four level-1 pixels of a 2x2 block share a lane quad, which then forms the level-2
+//    pixel from them. Levels 1 and 2 keep the averaged, renormalised normals like the CPU path.
 // Vectors are packed xyz float triples; invalid entries are zero.
 #include "common.hpp"
 
@@ -120,45 +120,55 @@ __device__ __forceinline__ Px down2x2(const Px s[4]) {
     return r;
 }
 
+// One thread per level-1 pixel: a block covers 16 x 4 of the 2x2 level-1 blocks, and the
+// four pixels of a block sit in one lane quad (lane 4 g + 2 j + i for pixel (i, j) of
+// block g), so the quad's first lane forms the level-2 pixel from the other three's
+// results without a round trip through memory.
+__device__ __forceinline__ Px shflPx(const Px& a, int src) {
+    Px r;
+    r.d = __shfl(a.d, src);
+    r.v = F3{__shfl(a.v.x, src), __shfl(a.v.y, src), __shfl(a.v.z, src)};
+    r.n = F3{__shfl(a.n.x, src), __shfl(a.n.y, src), __shfl(a.n.z, src)};
+    return r;
+}
+
 __global__ __launch_bounds__(256) void livePyramid(int W0, int H0, const float* __restrict__ d0,
                                                    const float* __restrict__ v0, const float* __restrict__ n0,
                                                    float* __restrict__ d1, float* __restrict__ v1,
                                                    float* __restrict__ n1, float* __restrict__ d2,
                                                    float* __restrict__ v2, float* __restrict__ n2) {
     const int W1 = W0 / 2, H1 = H0 / 2, W2 = W1 / 2, H2 = H1 / 2;
-    const int gx = blockIdx.x * 16 + (threadIdx.x & 15);  // 2x2 group of level-1 pixels
-    const int gy = blockIdx.y * 16 + (threadIdx.x >> 4);
-    if (2 * gx >= W1 || 2 * gy >= H1) return;
+    const int q = threadIdx.x & 3, g = threadIdx.x >> 2;
+    const int gx = blockIdx.x * 16 + (g & 15), gy = blockIdx.y * 4 + (g >> 4);  // 2x2 block of level 1
+    const int x1 = 2 * gx + (q & 1), y1 = 2 * gy + (q >> 1);
+    Px r{0.0f, {0, 0, 0}, {0, 0, 0}};
+    if (x1 < W1 && y1 < H1) {
+        Px s[4];
+#pragma unroll
+        for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+            for (int dx = 0; dx < 2; ++dx) {
+                const int si = (2 * y1 + dy) * W0 + (2 * x1 + dx);
+                s[dy * 2 + dx] = Px{d0[si], ld3(v0, si), ld3(n0, si)};
+            }
+        r = down2x2(s);
+        const int o = y1 * W1 + x1;
+        d1[o] = r.d;
+        st3(v1, o, r.v);
+        st3(n1, o, r.n);
+    }
+    // Every lane takes part in the exchange (pixels outside level 1 contribute zeros, as
+    // the level-2 pixel of a block reaching outside is not formed).
+    const int base = (int)(threadIdx.x & 63) & ~3;
     Px l1[4];
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-        for (int i = 0; i < 2; ++i) {
-            const int x1 = 2 * gx + i, y1 = 2 * gy + j;
-            Px r{0.0f, {0, 0, 0}, {0, 0, 0}};
-            if (x1 < W1 && y1 < H1) {
-                Px s[4];
-#pragma unroll
-                for (int dy = 0; dy < 2; ++dy)
-#pragma unroll
-                    for (int dx = 0; dx < 2; ++dx) {
-                        const int si = (2 * y1 + dy) * W0 + (2 * x1 + dx);
-                        s[dy * 2 + dx] = Px{d0[si], ld3(v0, si), ld3(n0, si)};
-                    }
-                r = down2x2(s);
-                const int o = y1 * W1 + x1;
-                d1[o] = r.d;
-                st3(v1, o, r.v);
-                st3(n1, o, r.n);
-            }
-            l1[j * 2 + i] = r;
-        }
-    if (gx < W2 && gy < H2) {
-        const Px r = down2x2(l1);
+    for (int k = 0; k < 4; ++k) l1[k] = shflPx(r, base + k);
+    if (q == 0 && gx < W2 && gy < H2) {
+        const Px r2 = down2x2(l1);
         const int o = gy * W2 + gx;
-        d2[o] = r.d;
-        st3(v2, o, r.v);
-        st3(n2, o, r.n);
+        d2[o] = r2.d;
+        st3(v2, o, r2.v);
+        st3(n2, o, r2.n);
     }
 }
 
@@ -170,7 +180,7 @@ hipError_t launchLive(bool mm, const void* src, int W, int H, Intr K, float min_
     else
         hipLaunchKernelGGL(liveLevel0<false>, g0, dim3(256), 0, st, src, W, H, K, min_d, max_d, d[0], v[0], n[0]);
     const int W1 = W / 2, H1 = H / 2;
-    const dim3 g1(((W1 + 1) / 2 + 15) / 16, ((H1 + 1) / 2 + 15) / 16);
+    const dim3 g1(((W1 + 1) / 2 + 15) / 16, ((H1 + 1) / 2 + 3) / 4);
     hipLaunchKernelGGL(livePyramid, g1, dim3(256), 0, st, W, H, d[0], v[0], n[0], d[1], v[1], n[1], d[2], v[2],
                        n[2]);
     return hipGetLastError();
@@ -182,7 +192,7 @@ namespace azu {
 
 hipError_t launchDownsample(int W, int H, float* const d[3], float* const v[3], float* const n[3], hipStream_t st) {
     const int W1 = W / 2, H1 = H / 2;
-    const dim3 g1(((W1 + 1) / 2 + 15) / 16, ((H1 + 1) / 2 + 15) / 16);
+    const dim3 g1(((W1 + 1) / 2 + 15) / 16, ((H1 + 1) / 2 + 3) / 4);
     hipLaunchKernelGGL(livePyramid, g1, dim3(256), 0, st, W, H, d[0], v[0], n[0], d[1], v[1], n[1], d[2], v[2],
                        n[2]);
     return hipGetLastError();
