@@ -13,8 +13,8 @@
 //   2. then, unless the level has stopped, every thread walks its live pixels, associates
 //      each with the model image, builds J = [R^T n_m ; v x R^T n_m], the Huber weight
 //      and loss, and accumulates the 21 upper-triangle A terms, 6 b terms and the
-//      residual in FP32 (fused multiply-adds over its few pixels), plus 6 integer
-//      counters; each wave sums its lanes by a fixed DPP tree, and the block adds its
+//      residual as Kahan-compensated FP32 partial sums, plus 6 integer counters; each
+//      wave sums its lanes' compensated sums by a fixed DPP tree, and the block adds its
 //      four wave sums in FP64 and writes one FP64 partial per slot.
 // The tracker state (pose, per-level result, stop flag) lives in device memory, so all
 // launches of a frame are enqueued back to back; once a level has stopped, its remaining
@@ -88,8 +88,8 @@ __device__ __forceinline__ float dppF(float v) {
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xf, true));
 }
 
-// Wave64 float sum to lane 63, the same DPP sequence as waveSumInt below (a fixed
-// tree of six adds per lane).
+// Wave64 float sum to lane 63, the same DPP sequence as waveSumInt below (a fixed tree
+// of six adds per lane, so at most six roundings between a thread's sum and the wave's).
 __device__ __forceinline__ float waveSumF(float v) {
     v += dppF<0xb1, 0xf>(v);
     v += dppF<0x4e, 0xf>(v);
@@ -98,6 +98,15 @@ __device__ __forceinline__ float waveSumF(float v) {
     v += dppF<0x142, 0xa>(v);
     v += dppF<0x143, 0xc>(v);
     return v;
+}
+
+// Kahan-compensated accumulation of one term into the sum s with compensation c (the
+// sum carried is s - c).
+__device__ __forceinline__ void kahanAdd(float& s, float& c, float term) {
+    const float y = term - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
 }
 
 // Wave64 integer sum to lane 63 with the gfx9 DPP sequence: quad_perm [1,0,3,2],
@@ -566,7 +575,7 @@ __device__ __forceinline__ int icpProject(const IcpIterArgs& a, const PoseDerive
 }
 
 __device__ __forceinline__ void icpAccumulate(const IcpIterArgs& a, const PoseDerived& pd, F3 lv, F3 mv, F3 mn,
-                                              F3 ln, float* acc, int* cnt) {
+                                              F3 ln, float* acc, float* comp, int* cnt) {
     if (!(f3finite(mv) && f3dot(mv, mv) > 1e-12f) || !(f3finite(mn) && f3dot(mn, mn) > 0.9f)) return;
     cnt[2]++;
     const F3 vlw = f3add(mulv(pd.R_cw, lv), pd.t_cw);
@@ -592,22 +601,19 @@ __device__ __forceinline__ void icpAccumulate(const IcpIterArgs& a, const PoseDe
     const float w = (ae <= 0.02f) ? 1.0f : 0.02f / ae;
     const float we = err * w;
     const float loss = (ae <= 0.02f) ? ae * ae : 2.0f * 0.02f * ae - 0.02f * 0.02f;
-    // A thread folds a handful of pixels (at most kPxBatch per level-0 pass of a
-    // 640 x 480 frame), and its wave 64 such sums, so the float sums stay within a few
-    // roundings of the terms' magnitudes; everything longer runs in FP64.
     int k = 0;
 #pragma unroll
     for (int i = 0; i < 6; ++i) {
         const float wj = w * J[i];
 #pragma unroll
         for (int j = i; j < 6; ++j) {
-            acc[k] = __builtin_fmaf(wj, J[j], acc[k]);
+            kahanAdd(acc[k], comp[k], wj * J[j]);
             ++k;
         }
     }
 #pragma unroll
-    for (int i = 0; i < 6; ++i) acc[21 + i] = __builtin_fmaf(-J[i], we, acc[21 + i]);
-    acc[27] += loss;
+    for (int i = 0; i < 6; ++i) kahanAdd(acc[21 + i], comp[21 + i], -(J[i] * we));
+    kahanAdd(acc[27], comp[27], loss);
     cnt[0]++;
 }
 
@@ -666,9 +672,12 @@ struct IcpShared {
 __device__ void icpBlockPass(const IcpIterArgs& a, const PoseDerived& pd, double* __restrict__ partials,
                              IcpShared& sh, const PassBatch& first) {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    float acc[kIcpSlotsF];
+    float acc[kIcpSlotsF], comp[kIcpSlotsF];
 #pragma unroll
-    for (int i = 0; i < kIcpSlotsF; ++i) acc[i] = 0.0f;
+    for (int i = 0; i < kIcpSlotsF; ++i) {
+        acc[i] = 0.0f;
+        comp[i] = 0.0f;
+    }
     int cnt[kIcpSlotsI] = {0, 0, 0, 0, 0, 0};
     const int iw = a.W - 2, ih = a.H - 2;
     const int npx = (iw > 0 && ih > 0) ? iw * ih : 0;
@@ -691,18 +700,19 @@ __device__ void icpBlockPass(const IcpIterArgs& a, const PoseDerived& pd, double
         }
 #pragma unroll
         for (int b = 0; b < kPxBatch; ++b)
-            if (midx[b] >= 0) icpAccumulate(a, pd, pb.lv[b], mv[b], mn[b], pb.ln[b], acc, cnt);
+            if (midx[b] >= 0) icpAccumulate(a, pd, pb.lv[b], mv[b], mn[b], pb.ln[b], acc, comp, cnt);
     }
 
-    // Block fold: a fixed DPP tree per wave in float (64 thread sums of a few terms
-    // each), then the four wave sums in FP64 in wave order. Counters are exact integers.
+    // Block fold: each thread's Kahan sum (s - c), a fixed DPP tree per wave, then the four
+    // wave sums in FP64 in wave order. Counters are exact integers.
+    float wsum[kIcpSlotsF];
 #pragma unroll
-    for (int i = 0; i < kIcpSlotsF; ++i) acc[i] = waveSumF(acc[i]);
+    for (int i = 0; i < kIcpSlotsF; ++i) wsum[i] = waveSumF(acc[i] - comp[i]);
 #pragma unroll
     for (int i = 0; i < kIcpSlotsI; ++i) cnt[i] = waveSumInt(cnt[i]);
     if (lane == 63) {
 #pragma unroll
-        for (int i = 0; i < kIcpSlotsF; ++i) sh.wsum[wv][i] = acc[i];
+        for (int i = 0; i < kIcpSlotsF; ++i) sh.wsum[wv][i] = wsum[i];
 #pragma unroll
         for (int i = 0; i < kIcpSlotsI; ++i) sh.cnt[wv][i] = cnt[i];
     }
