@@ -5,10 +5,10 @@
 // secant step and take the normal from six trilinear samples at +-vs.
 //
 // Layout: a 256-thread block owns a 16x16 pixel tile, each wave an 8x8 sub-tile, so
-// the rays of a wave march through neighbouring voxels. Blocks are remapped so that
-// each of the 8 XCDs (block b runs on XCD b % 8) handles one contiguous band of
-// tiles and its L2 holds the voxels of that band. Each trilinear sample reads its 8
-// corners as 4 x-adjacent pairs (16 B per pair for FP32, 8 B for FP16/U16).
+// the rays of a wave march through neighbouring voxels. For FP16 words the blocks are
+// remapped so that each of the 8 XCDs (block b runs on XCD b % 8) handles one
+// contiguous band of tiles; FP32 tiles run in launch order. Each trilinear sample reads
+// its 8 corners as 4 x-adjacent pairs (16 B per pair for FP32, 8 B for FP16/U16).
 //
 // Exact = true (FP32 parity path): IEEE division where the reference divides, no
 // contraction, bit-identical to the CPU restatement. Exact = false (FP16 form):
@@ -16,13 +16,6 @@
 #include <algorithm>
 
 #include "common.hpp"
-
-#ifndef RAY_SPEC32
-#define RAY_SPEC32 4
-#endif
-#ifndef RAY_SPEC16
-#define RAY_SPEC16 0
-#endif
 
 namespace azu {
 
@@ -85,10 +78,9 @@ struct RaySkip {
     const int* ok;
 };
 
-// sampleTSDF (TSDFVolume.cpp:492-525) in two halves, so a ray can issue the reads of
-// several samples before it evaluates the first: sampleLoad locates the cell and reads
-// its 8 corners (nothing is read outside the volume), sampleEval applies the weight
-// gate and interpolates.
+// sampleTSDF (TSDFVolume.cpp:492-525) in two halves: sampleLoad locates the cell and
+// reads its 8 corners (nothing is read outside the volume), sampleEval applies the
+// weight gate and interpolates.
 template <class S> struct Cell {
     typename PairIO<S>::P q0, q1, q2, q3;
     float tx, ty, tz;
@@ -239,8 +231,6 @@ struct RayOut {
     float* t_hit;  // optional: ray parameter of the hit (NaN on a miss)
 };
 
-// Far-step samples whose reads one ray issues together, per storage form.
-template <class S> constexpr int kSpecOf = S::kBytes == 8 ? RAY_SPEC32 : RAY_SPEC16;
 
 template <class S, bool E, int FD, bool B>
 __global__ __launch_bounds__(256) void raycastKernel(Vol<S> V, VolParams p, PoseArg pa, Intr K, int W, int H,
@@ -352,58 +342,6 @@ __global__ __launch_bounds__(256) void raycastKernel(Vol<S> V, VolParams p, Pose
         }
         // Once t_prev >= t_end every later sample is far: no hit can follow.
         while (go && t_prev < t_far && t_prev < t_end) {
-            if (kSpecOf<S> > 0 && (!prev_valid || f_prev >= 0.999f)) {
-                // A run of far steps: while every sample is unobserved or >= 0.999 the
-                // step stays far_step, so the next kSpec parameters follow from t_prev
-                // alone (the same recurrence as one step at a time). Their reads are
-                // issued together, then the samples are taken in order; the run ends at
-                // the first sample that changes the step, and the rest are discarded.
-                constexpr int kSpec = kSpecOf<S> > 0 ? kSpecOf<S> : 1;
-                float ts[kSpec];
-                Cell<S> cs[kSpec];
-                float tt = t_prev;
-#pragma unroll
-                for (int k = 0; k < kSpec; ++k) {
-                    tt = smin(tt + far_step, t_far);
-                    ts[k] = tt;
-                }
-#pragma unroll
-                for (int k = 0; k < kSpec; ++k) cs[k] = sampleLoad<S, E, FD, B>(V, R, f3add(cam, f3mul(dir, ts[k])));
-#pragma unroll
-                for (int k = 0; k < kSpec; ++k) {
-                    const float t = ts[k];
-                    float f = 1.0f;
-                    const bool valid = sampleEval<S, E>(cs[k], f);
-                    if (valid && !isfinite(f)) {
-                        go = false;
-                        break;
-                    }
-                    if (valid && prev_valid) {
-                        if (f_prev > 0.0f && f <= 0.0f) {
-                            t_hit = t_prev + (t - t_prev) * (f_prev / (f_prev - f));
-                            found = true;
-                            go = false;
-                            break;
-                        }
-                        if (f_prev < 0.0f && f >= 0.0f) {
-                            go = false;
-                            break;
-                        }
-                    } else if (valid && f < 0.0f) {
-                        go = false;
-                        break;
-                    }
-                    t_prev = t;
-                    f_prev = f;
-                    prev_valid = valid;
-                    if (t >= t_far) {
-                        go = false;
-                        break;
-                    }
-                    if (valid && f < 0.999f) break;
-                }
-                continue;
-            }
             const float step =
                 (!prev_valid || f_prev >= 0.999f) ? far_step : smax(min_step, 0.8f * f_prev * trunc);
             const float t = smin(t_prev + step, t_far);
