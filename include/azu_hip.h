@@ -237,8 +237,9 @@ int azu_tracker_pose_device(azu_tracker* t, const float** d_pose);
 /* Synchronises the stream and returns the result of the last track. */
 int azu_tracker_result(azu_tracker* t, azu_icp_result* out, azu_stream s);
 /* Diagnostic, summed over the ICP iterations that ran (not skipped after a level's
- * stop) since the last reset: out[0] device clock ms of the last block's cross-block
- * FP64 sum, out[1] of its solve + pose update, out[2] the number of such iterations,
+ * stop) since the last reset: out[0] device clock ms of block 0's cross-block FP64 sum
+ * (every block computes the same one), out[1] of its solve + pose update, out[2] the
+ * number of such iterations,
  * out[3] interior live pixels they visited, out[4] their in-bounds model projections.
  * Synchronises; reset != 0 clears. */
 int azu_tracker_phase_times(azu_tracker* t, double out[5], int reset);
