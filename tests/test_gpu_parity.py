@@ -98,3 +98,29 @@ def test_icp_single_frame_pose():
     # counts are taken at the two final poses, which differ by a few micrometres
     assert abs(rg["inliers"] - ro["inliers"]) <= 1e-3 * ro["inliers"]
     assert rg["tracking_ok"] == bool(ro["tracking_ok"])
+
+
+def test_icp_deterministic():
+    """Every block of a launch sums the previous launch's partials in one fixed order, so
+    the whole track is deterministic: two tracks of the same input agree bit for bit
+    (pose, information matrix, counts)."""
+    from azu_amd.fusion import ICPTracker
+    p = small_params(256, 0.004)
+    o = oracle_volume(p)
+    fr = frames(2)
+    (mm0, T0), (mm1, _) = fr
+    o.integrate(oracle.depth_mm(mm0), sc.K640, T0)
+    mv, mn = o.raycast(T0, sc.K640, 640, 480)
+    tr = ICPTracker()
+    dmm = N.device_array(mm1)
+    runs = []
+    for _ in range(2):
+        tr.prepare_mm_device(dmm.ptr)
+        runs.append(tr.track(mv, mn, T0, T0))
+    a, b = runs
+    say("two tracks: inliers", a["inliers"], b["inliers"], "steps", a["iterations"], b["iterations"])
+    assert np.array_equal(np.asarray(a["pose"], np.float32).view(np.uint32),
+                          np.asarray(b["pose"], np.float32).view(np.uint32))
+    assert np.array_equal(np.asarray(a["information"], np.float32).view(np.uint32),
+                          np.asarray(b["information"], np.float32).view(np.uint32))
+    assert a["inliers"] == b["inliers"] and a["iterations"] == b["iterations"]
