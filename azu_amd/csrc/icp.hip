@@ -732,9 +732,9 @@ __device__ void icpBlockPass(const IcpIterArgs& a, const PoseDerived& pd, double
 }
 
 // Cross-block sum, first step: thread (g, s) reads slot s of blocks g, g + 7, g + 14, ...
-// of the previous launch's partials (threads past 7 * 34: zeros). Reads
-// past the previous launch's nblk blocks go to the zero row kIcpMaxBlocks, which no
-// launch writes, so every read is unconditional and all are in flight at once.
+// of the previous launch's partials (threads past 7 * 34: zeros). Reads past the
+// previous launch's nblk blocks go to the zero row kIcpMaxBlocks, which no launch
+// writes, so every read is unconditional and all are in flight at once.
 constexpr int kMaxPer = (kIcpMaxBlocks + kSumGroups - 1) / kSumGroups;
 constexpr int kPartialRows = kIcpMaxBlocks + 1;
 __device__ __forceinline__ void loadPartials(const double* __restrict__ partials, int nblk, double* x) {
@@ -750,8 +750,8 @@ __device__ __forceinline__ void loadPartials(const double* __restrict__ partials
 }
 
 // The previous iteration's solve, in every block: FP64 cross-block sum of its partials
-// (gsum: the thread's loadPartials values added in block order), then the solve. The seven group sums are added in
-// group order. The order is fixed, so every block
+// (gsum: the thread's loadPartials values added in block order; the seven group sums
+// are then added in group order), then the solve. The order is fixed, so every block
 // gets the same sums, and the same solve. Returns true when the level's loop ends; the
 // new pose terms are left in sh.pd.
 __device__ bool blockSolve(IcpState* __restrict__ st, double gsum, IcpShared& sh, const PoseDerived& pd,
