@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "../../include/azu_hip.h"
+#include "../../include/tracking/TrackingPolicy.h"
 #include "common.hpp"
 
 namespace azu {
@@ -35,6 +36,7 @@ int icpBlocksFor(int W, int H);
 size_t icpPartialsBytes(int maxBlocks);
 hipError_t launchIcpInit(void* st, const PoseArg& est, const PoseArg& ref, hipStream_t s);
 hipError_t launchPredict(const float* cur, float* last, float* pred, hipStream_t s);
+hipError_t launchWritePose(void* st, const PoseArg& p, hipStream_t s);
 hipError_t launchIcpClearTicks(void* st, hipStream_t s);
 int depthLossBlocks(int W, int H);
 struct McResult {
@@ -866,6 +868,69 @@ int azu_tracker_predict(azu_tracker* t, const float** d_predicted, azu_stream s)
     AZU_HIP_TRY(launchPredict(static_cast<const float*>(t->state), t->motion, t->motion + 16, (hipStream_t)s),
                 "predict launch");
     if (d_predicted) *d_predicted = t->motion + 16;
+    return AZU_OK;
+}
+
+int azu_tracker_set_tracked_pose(azu_tracker* t, const float pose[16], azu_stream s) {
+    if (!t || !pose) return fail(AZU_ERR_ARG, "null argument");
+    AZU_HIP_TRY(launchWritePose(t->state, poseByValue(pose), (hipStream_t)s), "pose write launch");
+    return AZU_OK;
+}
+
+int azu_tracking_policy_defaults(azu_tracking_policy* p) {
+    if (!p) return fail(AZU_ERR_ARG, "null output pointer");
+    const kfusion::tracking::TrackingPolicy d;
+    p->min_inliers = d.min_inliers;
+    p->min_good_inliers = d.min_good_inliers;
+    p->min_fit_ratio = d.min_fit_ratio;
+    p->max_rms_m = d.max_rms_m;
+    p->max_frame_translation = d.max_frame_translation;
+    p->max_frame_rotation = d.max_frame_rotation;
+    p->failures_before_lost = d.failures_before_lost;
+    return AZU_OK;
+}
+
+int azu_track_grade(const azu_icp_result* r, const float prev_pose[16], const azu_tracking_policy* p,
+                    float degeneracy_rel, azu_track_decision* out) {
+    namespace T = kfusion::tracking;
+    if (!r || !prev_pose || !out) return fail(AZU_ERR_ARG, "null argument");
+    if (!(degeneracy_rel >= 0.0f) || !std::isfinite(degeneracy_rel))
+        return fail(AZU_ERR_ARG, "degeneracy_rel must be finite and >= 0");
+    T::TrackingPolicy pol;
+    if (p) {
+        pol.min_inliers = p->min_inliers;
+        pol.min_good_inliers = p->min_good_inliers;
+        pol.min_fit_ratio = p->min_fit_ratio;
+        pol.max_rms_m = p->max_rms_m;
+        pol.max_frame_translation = p->max_frame_translation;
+        pol.max_frame_rotation = p->max_frame_rotation;
+        pol.failures_before_lost = p->failures_before_lost;
+    }
+    T::ICPResult res;
+    std::memcpy(res.pose, r->pose, sizeof(res.pose));
+    std::memcpy(res.information, r->information, sizeof(res.information));
+    res.error = r->error;
+    res.inliers = r->inliers;
+    res.valid_live_points = r->valid_live_points;
+    res.valid_model_points = r->valid_model_points;
+    int dofs = 0;
+    if (degeneracy_rel > 0.0f) {
+        const T::ObservableMotion om = T::keepObservableMotion(prev_pose, res.pose, res.information, degeneracy_rel);
+        std::memcpy(res.pose, om.pose, sizeof(res.pose));
+        dofs = om.degenerate_dofs;
+    }
+    const T::TrackQuality q = T::classifyTracking(res, prev_pose, pol);
+    out->quality = q == T::TrackQuality::Good ? AZU_TRACK_GOOD
+                                              : (q == T::TrackQuality::Poor ? AZU_TRACK_POOR : AZU_TRACK_FAILED);
+    out->degenerate_dofs = dofs;
+    out->integrate = q == T::TrackQuality::Good ? 1 : 0;
+    std::memcpy(out->pose, q == T::TrackQuality::Failed ? prev_pose : res.pose, sizeof(out->pose));
+    return AZU_OK;
+}
+
+int azu_track_weakest_ratio(const float information[36], float* out) {
+    if (!information || !out) return fail(AZU_ERR_ARG, "null argument");
+    *out = kfusion::tracking::weakestDirectionRatio(information);
     return AZU_OK;
 }
 

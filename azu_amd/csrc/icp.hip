@@ -878,6 +878,17 @@ hipError_t launchIcpInit(void* st, const PoseArg& est, const PoseArg& ref, hipSt
     return hipGetLastError();
 }
 
+// Overwrites the tracked pose (the state's first 16 floats) with a host pose passed by
+// value, leaving the motion model's previous pose and the last result's other fields.
+__global__ void writeTrackedPose(float* __restrict__ dst, PoseArg p) {
+    if (blockIdx.x == 0 && threadIdx.x < 16) dst[threadIdx.x] = p.m[threadIdx.x];
+}
+
+hipError_t launchWritePose(void* st, const PoseArg& p, hipStream_t s) {
+    hipLaunchKernelGGL(writeTrackedPose, dim3(1), dim3(64), 0, s, static_cast<float*>(st), p);
+    return hipGetLastError();
+}
+
 hipError_t launchPredict(const float* cur, float* last, float* pred, hipStream_t s) {
     hipLaunchKernelGGL(predictPose, dim3(1), dim3(64), 0, s, cur, last, pred);
     return hipGetLastError();

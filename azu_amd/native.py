@@ -59,6 +59,30 @@ class IcpResult(ctypes.Structure):
     ]
 
 
+class TrackingPolicy(ctypes.Structure):
+    _fields_ = [
+        ("min_inliers", ctypes.c_int),
+        ("min_good_inliers", ctypes.c_int),
+        ("min_fit_ratio", ctypes.c_float),
+        ("max_rms_m", ctypes.c_float),
+        ("max_frame_translation", ctypes.c_float),
+        ("max_frame_rotation", ctypes.c_float),
+        ("failures_before_lost", ctypes.c_int),
+    ]
+
+
+class TrackDecision(ctypes.Structure):
+    _fields_ = [
+        ("quality", ctypes.c_int),
+        ("degenerate_dofs", ctypes.c_int),
+        ("integrate", ctypes.c_int),
+        ("pose", ctypes.c_float * 16),
+    ]
+
+
+TRACK_GOOD, TRACK_POOR, TRACK_FAILED = 0, 1, 2
+
+
 class AzuError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__("azu status %d: %s" % (code, msg))
@@ -130,6 +154,11 @@ _SIGS = {
                                                c_void_p, c_void_p]),
     "azu_tracker_set_pose": (ctypes.c_int, [c_void_p, c_float_p, c_void_p]),
     "azu_tracker_predict": (ctypes.c_int, [c_void_p, ctypes.POINTER(c_void_p), c_void_p]),
+    "azu_tracker_set_tracked_pose": (ctypes.c_int, [c_void_p, c_float_p, c_void_p]),
+    "azu_tracking_policy_defaults": (ctypes.c_int, [ctypes.POINTER(TrackingPolicy)]),
+    "azu_track_grade": (ctypes.c_int, [ctypes.POINTER(IcpResult), c_float_p, ctypes.POINTER(TrackingPolicy),
+                                       ctypes.c_float, ctypes.POINTER(TrackDecision)]),
+    "azu_track_weakest_ratio": (ctypes.c_int, [c_float_p, ctypes.POINTER(ctypes.c_float)]),
     "azu_tracker_pose_device": (ctypes.c_int, [c_void_p, ctypes.POINTER(c_void_p)]),
     "azu_tracker_result": (ctypes.c_int, [c_void_p, ctypes.POINTER(IcpResult), c_void_p]),
     "azu_tracker_phase_times": (ctypes.c_int, [c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_int]),

@@ -88,6 +88,28 @@ typedef struct azu_icp_result {
     int iterations; /* accepted steps, one byte per level (level l at bit 8*l) */
 } azu_icp_result;
 
+/* Frame-level tracking grade of kfusion::tracking::TrackQuality (TrackingPolicy.h:36). */
+typedef enum azu_track_quality { AZU_TRACK_GOOD = 0, AZU_TRACK_POOR = 1, AZU_TRACK_FAILED = 2 } azu_track_quality;
+
+/* Mirrors kfusion::tracking::TrackingPolicy (include/tracking/TrackingPolicy.h:40-48). */
+typedef struct azu_tracking_policy {
+    int min_inliers;             /* below this: Failed */
+    int min_good_inliers;        /* Good needs this much support */
+    float min_fit_ratio;         /* inliers / valid_model_points */
+    float max_rms_m;             /* sqrt(error) */
+    float max_frame_translation; /* m between consecutive frames */
+    float max_frame_rotation;    /* rad */
+    int failures_before_lost;
+} azu_tracking_policy;
+
+/* One graded ICP result: what the pipeline does with the frame. */
+typedef struct azu_track_decision {
+    int quality;         /* azu_track_quality */
+    int degenerate_dofs; /* directions keepObservableMotion dropped (0 when it is off) */
+    int integrate;       /* 1 for Good: update the pose and integrate */
+    float pose[16];      /* the pose to continue from: the graded pose, or prev_pose when Failed */
+} azu_track_decision;
+
 typedef struct azu_volume azu_volume;
 typedef struct azu_tracker azu_tracker;
 
@@ -232,6 +254,25 @@ int azu_tracker_set_pose(azu_tracker* t, const float pose[16], azu_stream s);
  * current is the tracker's device pose. *d_predicted receives the device pointer of the
  * prediction (16 floats), to pass as the estimate of azu_tracker_track_dpose. */
 int azu_tracker_predict(azu_tracker* t, const float** d_predicted, azu_stream s);
+/* Sets the tracked pose (the one azu_tracker_pose_device points at and
+ * azu_tracker_predict extrapolates from) without touching the motion model's previous
+ * pose: after a graded track, the pose the pipeline continues from
+ * (PipelineController.cpp:1500-1590 sets current_pose_ only). */
+int azu_tracker_set_tracked_pose(azu_tracker* t, const float pose[16], azu_stream s);
+/* The reference's TrackingPolicy defaults (TrackingPolicy.h:40-48): 100, 2000, 0.40,
+ * 0.015 m, 0.15 m, 0.52 rad, 3. */
+int azu_tracking_policy_defaults(azu_tracking_policy* p);
+/* Grades one ICP result the way the reference pipeline does while tracking
+ * (PipelineController.cpp:1370-1406): when degeneracy_rel > 0 the pose is first replaced
+ * by keepObservableMotion(prev_pose, r->pose, r->information, degeneracy_rel)
+ * (TrackingPolicy.h:126-161; the pipeline's default ratio is 5e-3), then graded by
+ * classifyTracking against prev_pose (TrackingPolicy.h:93-99). p NULL: the defaults.
+ * Host computation only. */
+int azu_track_grade(const azu_icp_result* r, const float prev_pose[16], const azu_tracking_policy* p,
+                    float degeneracy_rel, azu_track_decision* out);
+/* weakestDirectionRatio (TrackingPolicy.h:104-112): smallest over largest eigenvalue of
+ * a 6x6 information matrix, 0 for an empty, non-finite or unobserved direction. */
+int azu_track_weakest_ratio(const float information[36], float* out);
 /* Device pointer to the tracked pose (16 floats), valid once the enqueued track ran. */
 int azu_tracker_pose_device(azu_tracker* t, const float** d_pose);
 /* Synchronises the stream and returns the result of the last track. */

@@ -67,6 +67,7 @@ def test_argument_errors_without_device():
 CPP_LOOP = r"""
 #include "tsdf/TSDFVolume.h"
 #include "tracking/ICPTracker.h"
+#include "tracking/TrackingPolicy.h"
 #include "meshing/MarchingCubes.h"
 #include <cstdio>
 struct Mat4 { float m[16]; const float* data() const { return m; } };
@@ -82,12 +83,18 @@ int run(const float* d_depth, int W, int H, int frames) {
     tracking::ModelFrame model(W, H);
     Mat4 pose{{1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1}};
     for (int f = 0; f < frames; ++f) {
+        bool integrate = true;
         if (f > 0) {
             model.render(vol, pose, k);
             tracking::ICPResult r = icp.trackGPU(d_depth, nullptr, W, H, model, pose, model.pose);
-            if (r.tracking_ok) for (int i = 0; i < 16; ++i) pose.m[i] = r.pose[i];
+            const tracking::TrackQuality q = tracking::classifyTracking(r, pose);
+            if (q != tracking::TrackQuality::Failed) {
+                const tracking::ObservableMotion m = tracking::keepObservableMotion(pose.m, r.pose, r.information);
+                for (int i = 0; i < 16; ++i) pose.m[i] = m.pose[i];
+            }
+            integrate = q == tracking::TrackQuality::Good;
         }
-        vol.integrateGPU(d_depth, nullptr, pose, k.fx, k.fy, k.cx, k.cy, W, H);
+        if (integrate) vol.integrateGPU(d_depth, nullptr, pose, k.fx, k.fy, k.cx, k.cy, W, H);
     }
     meshing::MarchingCubes mc;
     auto mesh = mc.extract(vol, [](float x) { std::printf("%f\n", x); });
