@@ -11,7 +11,8 @@
 The tracked pose never leaves the device inside the loop (the ICP state feeds
 integrate and raycast through a device pointer); each frame's pose is copied to a
 pinned host slot asynchronously, so frames are enqueued back to back with no host
-synchronisation. Grading, retry and relocalisation are not part of the loop.
+synchronisation. Grading and retry are GradedFrameLoop's (one host synchronisation per
+track); relocalisation is not part of either loop.
 """
 import ctypes
 
@@ -89,3 +90,105 @@ class FrameLoop:
             self._poses_host = None
         self.tr.close()
         self.vol.close()
+
+
+class GradedFrameLoop(FrameLoop):
+    """The frame loop with the reference pipeline's tracking decision
+    (src/app/PipelineController.cpp:1160-1187, 1329-1406, 1500-1590), graded on the host
+    by azu_track_grade (include/tracking/TrackingPolicy.h):
+
+      estimate  the previous pose, or the constant-velocity prediction;
+      track     against the model image with the model's raycast pose as reference; when
+                the estimate was a prediction and the result grades Failed, track the
+                same prepared live frame again from the previous pose (the retry);
+      grade     keepObservableMotion (degeneracy_rel > 0), then classifyTracking;
+      Good      continue from the graded pose, integrate, raycast the model there;
+      Poor      continue from the graded pose; volume and model image unchanged;
+      Failed    keep the previous pose; failures_before_lost consecutive failures set
+                `lost` (relocalisation is not part of this loop).
+
+    Each frame synchronises once per track to read the ICP result, as the reference's
+    tracking thread does; FrameLoop is the unsynchronised loop without grading."""
+
+    def __init__(self, *args, policy=None, degeneracy_rel=5e-3, **kw):
+        super().__init__(*args, **kw)
+        self.policy = policy or N.TrackingPolicy()
+        if policy is None:
+            check(lib.azu_tracking_policy_defaults(ctypes.byref(self.policy)))
+        self.degeneracy_rel = float(degeneracy_rel)
+        self.d_model_pose = N.DeviceBuffer(64)
+        self.d_prev = N.DeviceBuffer(64)
+        self.decisions = []
+        self.consecutive_failures = 0
+        self.lost = False
+        self._prev = None
+
+    def reset(self, pose0):
+        super().reset(pose0)
+        self._prev = (ctypes.c_float * 16)(*N.f32(pose_cm(pose0)).tolist())
+        self.decisions = []
+        self.consecutive_failures = 0
+        self.lost = False
+
+    def _track(self, d_est):
+        check(lib.azu_tracker_track_dpose(self.tr.h, self.mv.ptr, self.mn.ptr, self.W, self.H, d_est,
+                                          self.d_model_pose.ptr, self.stream))
+        r = N.IcpResult()
+        check(lib.azu_tracker_result(self.tr.h, ctypes.byref(r), self.stream))
+        return r
+
+    def _grade(self, r, rel):
+        d = N.TrackDecision()
+        check(lib.azu_track_grade(ctypes.byref(r), self._prev, ctypes.byref(self.policy), rel, ctypes.byref(d)))
+        return d
+
+    def frame(self, d_mm):
+        """Run one frame from device uint16 millimetres; returns its decision record."""
+        s = self.stream
+        check(lib.azu_tracker_prepare_mm(self.tr.h, d_mm, s))
+        rec = dict(quality=N.TRACK_GOOD, degenerate_dofs=0, retried=False, inliers=0, weakest_ratio=None)
+        integrate = True
+        if self.nframes > 0:
+            check(lib.azu_memcpy_d2d(self.d_prev.ptr, self.d_pose, 64, s))
+            est, predicted = self.d_pose, False
+            if self.motion_model == "constant_velocity":
+                pred = ctypes.c_void_p()
+                check(lib.azu_tracker_predict(self.tr.h, ctypes.byref(pred), s))
+                est, predicted = pred.value, True
+            r = self._track(est)
+            if predicted and self._grade(r, 0.0).quality == N.TRACK_FAILED:
+                r = self._track(self.d_prev.ptr)
+                rec["retried"] = True
+            d = self._grade(r, self.degeneracy_rel)
+            check(lib.azu_tracker_set_tracked_pose(self.tr.h, d.pose, s))
+            self._prev = (ctypes.c_float * 16)(*d.pose)
+            ratio = ctypes.c_float()
+            check(lib.azu_track_weakest_ratio(r.information, ctypes.byref(ratio)))
+            rec.update(quality=d.quality, degenerate_dofs=d.degenerate_dofs, inliers=r.inliers,
+                       weakest_ratio=ratio.value)
+            integrate = bool(d.integrate)
+            if d.quality == N.TRACK_FAILED:
+                self.consecutive_failures += 1
+                if self.consecutive_failures >= self.policy.failures_before_lost:
+                    self.lost = True
+            else:
+                self.consecutive_failures = 0
+        if integrate:
+            lo, hi = self.vol.params.min_depth, self.vol.params.max_depth
+            check(lib.azu_integrate_dpose(self.vol.h, self.d0, self.W, self.H, N.fptr(self.K), self.d_pose, lo, hi,
+                                          s))
+            check(lib.azu_raycast_dpose(self.vol.h, self.d_pose, N.fptr(self.K), self.W, self.H, self.mv.ptr,
+                                        self.mn.ptr, None, s))
+            check(lib.azu_memcpy_d2d(self.d_model_pose.ptr, self.d_pose, 64, s))
+        if self.nframes < self.max_frames:
+            check(lib.azu_memcpy_d2h(self._poses_host + 64 * self.nframes, self.d_pose, 64, s))
+        self.nframes += 1
+        self.decisions.append(rec)
+        return rec
+
+    def close(self):
+        for b in (self.d_model_pose, self.d_prev):
+            if b is not None:
+                b.free()
+        self.d_model_pose = self.d_prev = None
+        super().close()

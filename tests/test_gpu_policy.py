@@ -1,0 +1,150 @@
+"""The graded frame loop on the GPU (azu_amd.pipeline.GradedFrameLoop: the reference
+pipeline's tracking decision, src/app/PipelineController.cpp:1160-1187, 1329-1406,
+1500-1590, over include/tracking/TrackingPolicy.h).
+
+  * every frame Good and the degeneracy filter off: the graded loop tracks bit for bit like the
+    ungraded loop with the same motion model (grading changes nothing it should not);
+  * a blank depth frame grades Failed after its retry: the pose is held bit for bit, the
+    volume is not integrated, tracking resumes on the next frames; three in a row set
+    `lost`;
+  * the retry tracks the same prepared live frame again: a solve from a far estimate
+    grades Failed, and the solve from the previous pose that follows equals a solve of
+    a freshly prepared frame bit for bit.
+The workload is the 512^3 / 2 mm sphere + plane + 10 boxes sequence with the motion model
+on."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from azu_amd import native as N
+from azu_amd import scene as sc
+from azu_amd.native import check, lib
+from azu_amd.pipeline import FrameLoop, GradedFrameLoop
+from tests.gpu_util import pose_diff, say
+
+pytestmark = pytest.mark.gpu
+
+NF = 10
+
+
+@pytest.fixture(scope="module")
+def seq():
+    frames, truth, K = sc.sequence("c2_512", frames=NF)
+    blank = N.device_array(np.zeros_like(frames[0]))
+    return truth, K, [N.device_array(f) for f in frames], blank
+
+
+def graded(K, **kw):
+    return GradedFrameLoop(sc.volume_params("c2_512"), N.FMT_F32, 640, 480, K, motion_model="constant_velocity",
+                           **kw)
+
+
+def test_graded_loop_equals_ungraded_when_every_frame_is_good(seq):
+    truth, K, dev, _ = seq
+    say("graded vs ungraded loop, %d frames, constant velocity" % NF)
+    plain = FrameLoop(sc.volume_params("c2_512"), N.FMT_F32, 640, 480, K, motion_model="constant_velocity")
+    plain.reset(truth[0])
+    for d in dev:
+        plain.frame(d.ptr)
+    P0 = plain.poses()
+    plain.close()
+    loop = graded(K, degeneracy_rel=0.0)
+    loop.reset(truth[0])
+    for d in dev:
+        loop.frame(d.ptr)
+    P1 = loop.poses()
+    dec = loop.decisions
+    loop.close()
+    assert all(r["quality"] == N.TRACK_GOOD and not r["retried"] for r in dec), dec
+    assert np.array_equal(P0, P1)
+    err0 = max(pose_diff(P1[k], truth[k])[0] for k in range(NF))
+    say("  no degeneracy filter: all Good, poses identical to the ungraded loop, max |t - t_true| %.2f mm"
+        % (1000 * err0))
+    # The default degeneracy filter (ratio 5e-3).
+    loop = graded(K)
+    loop.reset(truth[0])
+    for d in dev:
+        loop.frame(d.ptr)
+    P2 = loop.poses()
+    dec = loop.decisions
+    loop.close()
+    errs = [pose_diff(P2[k], truth[k]) for k in range(NF)]
+    for k, r in enumerate(dec[1:], 1):
+        say("  frame %d: %s dofs %d weakest ratio %.2e |t - t_true| %.2f mm %.3f deg"
+            % (k, r["quality"], r["degenerate_dofs"], r["weakest_ratio"], 1000 * errs[k][0], np.degrees(errs[k][1])))
+    assert all(r["quality"] == N.TRACK_GOOD for r in dec), dec
+    for k, r in enumerate(dec[1:], 1):
+        assert (r["degenerate_dofs"] > 0) == (r["weakest_ratio"] < 5e-3)
+        if r["degenerate_dofs"] == 0 and all(q["degenerate_dofs"] == 0 for q in dec[:k]):
+            assert np.array_equal(P2[k], P0[k])
+
+
+def test_blank_frame_fails_holds_pose_and_skips_integration(seq):
+    truth, K, dev, blank = seq
+    say("graded loop with a blank depth frame at frame 5")
+    loop = graded(K, degeneracy_rel=0.0)
+    loop.reset(truth[0])
+    for k in range(NF):
+        if k == 5:
+            loop.vol.counters(reset=True)
+            rec = loop.frame(blank.ptr)
+            updated, _ = loop.vol.counters(reset=True)
+            assert rec["quality"] == N.TRACK_FAILED and rec["retried"], rec
+            assert rec["inliers"] < 100
+            assert updated == 0
+            assert loop.consecutive_failures == 1 and not loop.lost
+        else:
+            rec = loop.frame(dev[k].ptr)
+            assert rec["quality"] == N.TRACK_GOOD, (k, rec)
+    P = loop.poses()
+    assert loop.consecutive_failures == 0 and not loop.lost
+    loop.close()
+    assert np.array_equal(P[5], P[4])
+    err = pose_diff(P[NF - 1], truth[NF - 1])[0]
+    say("  frame 5 Failed (retried), pose held, no voxel updated; final |t - t_true| %.2f mm" % (1000 * err))
+    assert err < 2e-3
+
+
+def test_three_failures_set_lost(seq):
+    truth, K, dev, blank = seq
+    loop = graded(K, degeneracy_rel=0.0)
+    loop.reset(truth[0])
+    for k in range(3):
+        loop.frame(dev[k].ptr)
+    for _ in range(3):
+        assert loop.frame(blank.ptr)["quality"] == N.TRACK_FAILED
+    assert loop.consecutive_failures == 3 and loop.lost
+    P = loop.poses()
+    loop.close()
+    assert np.array_equal(P[3], P[2]) and np.array_equal(P[5], P[2])
+
+
+def test_retry_reuses_the_prepared_live_frame(seq):
+    truth, K, dev, _ = seq
+    loop = graded(K)
+    loop.reset(truth[0])
+    for k in range(4):
+        loop.frame(dev[k].ptr)
+    h, s = loop.tr.h, None
+    prev = np.array(list(loop._prev), np.float32)
+    far = prev.copy()
+    far[12] += 0.5  # 0.5 m along x (column-major translation)
+    mv, mn = loop.mv.ptr, loop.mn.ptr
+
+    def track(est):
+        check(lib.azu_tracker_track(h, mv, mn, 640, 480, N.fptr(est), N.fptr(prev), s))
+        r = N.IcpResult()
+        check(lib.azu_tracker_result(h, ctypes.byref(r), s))
+        return r
+
+    check(lib.azu_tracker_prepare_mm(h, dev[4].ptr, s))
+    r_far = track(far)
+    assert loop._grade(r_far, 0.0).quality == N.TRACK_FAILED
+    r_retry = track(prev)
+    check(lib.azu_tracker_prepare_mm(h, dev[4].ptr, s))
+    r_fresh = track(prev)
+    loop.close()
+    assert np.array_equal(np.array(r_retry.pose), np.array(r_fresh.pose))
+    assert r_retry.inliers == r_fresh.inliers and r_retry.iterations == r_fresh.iterations
+    say("  far estimate: %d inliers -> Failed; retry == fresh solve (%d inliers)" % (r_far.inliers, r_retry.inliers))
