@@ -46,6 +46,8 @@ struct McResult {
     float* nrm;
     uint32_t* idx;
 };
+hipError_t launchPointCloud(int format, const void* vox, const VolParams& p, float* d_out,
+                            unsigned long long capacity, unsigned long long* total, hipStream_t st);
 hipError_t launchMarchingCubes(int format, const void* vox, const VolParams& p, unsigned long long max_tris,
                                McResult* res, hipStream_t st);
 hipError_t launchDepthLoss(int format, const void* vox, const VolParams& p, const PoseArg& pa, Intr K, int W, int H,
@@ -648,6 +650,18 @@ int azu_raycast_depth_loss(azu_volume* v, const float pose[16], const float K[4]
 struct azu_mesh {
     McResult r;
 };
+
+int azu_vol_point_cloud(azu_volume* v, float* d_points, unsigned long long capacity, unsigned long long* count,
+                        azu_stream s) {
+    if (!v || !count) return fail(AZU_ERR_ARG, "null volume or count");
+    if (capacity > 0 && !d_points) return fail(AZU_ERR_ARG, "null point buffer with a non-zero capacity");
+    if ((size_t)v->params.resolution * v->params.resolution * v->params.resolution >= ((size_t)1 << 31))
+        return fail(AZU_ERR_ARG, "point cloud: resolution^3 must stay below 2^31");
+    ProfScope ps("point_cloud", (hipStream_t)s);
+    AZU_HIP_TRY(launchPointCloud(v->format, v->vox, toVol(v->params), d_points, capacity, count, (hipStream_t)s),
+                "point cloud");
+    return AZU_OK;
+}
 
 int azu_mc_extract(azu_volume* v, unsigned long long max_triangles, azu_mesh** out, azu_stream s) {
     if (!v || !out) return fail(AZU_ERR_ARG, "null volume or output");

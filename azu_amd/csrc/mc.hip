@@ -354,6 +354,58 @@ __global__ __launch_bounds__(256) void writeMesh(McVol<S> V, const uint32_t* __r
     }
 }
 
+// ---------------------------------------------------------------------------
+// Global point cloud (TSDFVolume::extractGlobalPointCloud, TSDFVolume.cpp:571-632, and
+// its GPU form compactPointsKernel, TSDFVolume_cuda.cu:409-447): every voxel with
+// weight > 1 and |tsdf| < 0.2, as origin + (x, y, z) * voxel_size, in linear voxel
+// order. One wave per x-row of the volume: the row count is a ballot popcount per 64
+// voxels; the write pass places each point at its row's offset plus the lane's rank.
+
+template <class S>
+__device__ __forceinline__ bool pcKeep(typename S::Word w) {
+    return S::weight(w) > 1.0f && fabsf(S::tsdf(w)) < 0.2f;
+}
+
+template <class S>
+__global__ __launch_bounds__(256) void pointCount(McVol<S> V, uint32_t* __restrict__ row_count) {
+    const size_t row = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= (size_t)V.R * V.R) return;
+    const typename S::Word* src = V.vox + row * V.R;
+    uint32_t n = 0;
+    for (int x0 = 0; x0 < V.R; x0 += 64) {
+        const int x = x0 + lane;
+        const bool keep = x < V.R && pcKeep<S>(src[x]);
+        n += (uint32_t)__popcll(__ballot(keep));
+    }
+    if (lane == 0) row_count[row] = n;
+}
+
+template <class S>
+__global__ __launch_bounds__(256) void pointWrite(McVol<S> V, const uint32_t* __restrict__ row_off,
+                                                  unsigned long long capacity, float* __restrict__ out) {
+    const size_t row = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= (size_t)V.R * V.R) return;
+    const int z = (int)(row / (size_t)V.R), y = (int)(row % (size_t)V.R);
+    const float py = V.oy + (float)y * V.vs, pz = V.oz + (float)z * V.vs;
+    const typename S::Word* src = V.vox + row * V.R;
+    unsigned long long base = row_off[row];
+    for (int x0 = 0; x0 < V.R; x0 += 64) {
+        const int x = x0 + lane;
+        const bool keep = x < V.R && pcKeep<S>(src[x]);
+        const unsigned long long mask = __ballot(keep);
+        const unsigned rank = __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+        const unsigned long long k = base + rank;
+        if (keep && k < capacity) {
+            out[3 * k + 0] = V.ox + (float)x * V.vs;
+            out[3 * k + 1] = py;
+            out[3 * k + 2] = pz;
+        }
+        base += (unsigned long long)__popcll(mask);
+    }
+}
+
 }  // namespace mc
 
 struct McResult {
@@ -485,6 +537,54 @@ hipError_t launchMarchingCubes(int format, const void* vox, const VolParams& p, 
     }
     mc::McVol<StoreF16> V{static_cast<const uint32_t*>(vox), p.res, p.vs, p.origin[0], p.origin[1], p.origin[2]};
     return mcRun(V, max_tris, res, st);
+}
+
+template <class S>
+static hipError_t pcRun(const mc::McVol<S>& V, float* d_out, unsigned long long capacity, unsigned long long* total,
+                        hipStream_t st) {
+    *total = 0;
+    const size_t rows = (size_t)V.R * V.R;
+    if (V.R < 1) return hipSuccess;
+    if (rows + 1 >= ((size_t)1 << 31)) return hipErrorInvalidValue;
+    uint32_t *cnt = nullptr, *off = nullptr;
+    void* tmp = nullptr;
+    size_t tmp_bytes = 0;
+    hipError_t e = hipMalloc(&cnt, (rows + 1) * 4);
+    if (e == hipSuccess) e = hipMalloc(&off, (rows + 1) * 4);
+    const unsigned blocks = (unsigned)((rows + 3) / 4);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(mc::pointCount<S>, dim3(blocks), dim3(256), 0, st, V, cnt);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemsetAsync(cnt + rows, 0, 4, st);
+    // exclusive scan over rows + 1 entries: off[rows] is the total (below 2^32: at most
+    // 2^31 voxels)
+    if (e == hipSuccess) e = hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, cnt, off, (int)(rows + 1), st);
+    if (e == hipSuccess) e = hipMalloc(&tmp, tmp_bytes);
+    if (e == hipSuccess) e = hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, cnt, off, (int)(rows + 1), st);
+    if (e == hipSuccess && d_out && capacity > 0) {
+        hipLaunchKernelGGL(mc::pointWrite<S>, dim3(blocks), dim3(256), 0, st, V, off, capacity, d_out);
+        e = hipGetLastError();
+    }
+    uint32_t n = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&n, off + rows, 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e == hipSuccess) *total = n;
+    if (cnt) (void)hipFree(cnt);
+    if (off) (void)hipFree(off);
+    if (tmp) (void)hipFree(tmp);
+    return e;
+}
+
+hipError_t launchPointCloud(int format, const void* vox, const VolParams& p, float* d_out,
+                            unsigned long long capacity, unsigned long long* total, hipStream_t st) {
+    if ((size_t)p.res * p.res * p.res >= ((size_t)1 << 31)) return hipErrorInvalidValue;
+    if (format == kFmtF32) {
+        mc::McVol<StoreF32> V{static_cast<const float2*>(vox), p.res, p.vs, p.origin[0], p.origin[1], p.origin[2]};
+        return pcRun(V, d_out, capacity, total, st);
+    }
+    mc::McVol<StoreF16> V{static_cast<const uint32_t*>(vox), p.res, p.vs, p.origin[0], p.origin[1], p.origin[2]};
+    return pcRun(V, d_out, capacity, total, st);
 }
 
 }  // namespace azu

@@ -160,6 +160,23 @@ class TSDFVolume:
             lib.azu_mesh_destroy(m.value)
         return pos, nrm, idx, bool(tr.value)
 
+    def point_cloud(self):
+        """TSDFVolume::extractGlobalPointCloud on the device -> points (P, 3) float32: the
+        voxels with weight > 1 and |tsdf| < 0.2 at origin + (x, y, z) * voxel_size, in
+        linear voxel order (x fastest)."""
+        n = ctypes.c_ulonglong()
+        check(lib.azu_vol_point_cloud(self.h, None, 0, ctypes.byref(n), None))
+        if n.value == 0:
+            return np.zeros((0, 3), np.float32)
+        buf = N.DeviceBuffer(n.value * 12)
+        try:
+            m = ctypes.c_ulonglong()
+            check(lib.azu_vol_point_cloud(self.h, buf.ptr, n.value, ctypes.byref(m), None))
+            assert m.value == n.value
+            return buf.download(np.float32, (n.value, 3))
+        finally:
+            buf.free()
+
     def storage(self):
         p = ctypes.c_void_p()
         b = ctypes.c_size_t()

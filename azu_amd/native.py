@@ -155,6 +155,8 @@ _SIGS = {
     "azu_tracker_set_pose": (ctypes.c_int, [c_void_p, c_float_p, c_void_p]),
     "azu_tracker_predict": (ctypes.c_int, [c_void_p, ctypes.POINTER(c_void_p), c_void_p]),
     "azu_tracker_set_tracked_pose": (ctypes.c_int, [c_void_p, c_float_p, c_void_p]),
+    "azu_vol_point_cloud": (ctypes.c_int, [c_void_p, c_void_p, ctypes.c_ulonglong,
+                                           ctypes.POINTER(ctypes.c_ulonglong), c_void_p]),
     "azu_tracking_policy_defaults": (ctypes.c_int, [ctypes.POINTER(TrackingPolicy)]),
     "azu_track_grade": (ctypes.c_int, [ctypes.POINTER(IcpResult), c_float_p, ctypes.POINTER(TrackingPolicy),
                                        ctypes.c_float, ctypes.POINTER(TrackDecision)]),

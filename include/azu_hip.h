@@ -169,6 +169,14 @@ int azu_vol_storage(azu_volume* v, void** d_ptr, size_t* bytes);
  * (reset != 0 clears them after reading). Synchronises. */
 int azu_vol_counters(azu_volume* v, unsigned long long* updated, unsigned long long* newly, int reset);
 int azu_vol_integrated_frames(const azu_volume* v, int* frames);
+/* TSDFVolume::extractGlobalPointCloud (TSDFVolume.cpp:571-632; its GPU form
+ * compactPointsKernel, TSDFVolume_cuda.cu:409-447): every voxel with weight > 1 and
+ * |tsdf| < 0.2, as origin + (x, y, z) * voxel_size, xyz float triplets in linear voxel
+ * order (x fastest). Writes min(count, capacity) points to d_points (device memory for
+ * capacity * 3 floats; NULL with capacity 0 queries the count) and the full count to
+ * *count. Colour is not stored, so no colours are produced. Synchronises. */
+int azu_vol_point_cloud(azu_volume* v, float* d_points, unsigned long long capacity, unsigned long long* count,
+                        azu_stream s);
 
 /* TSDFVolume::integrate / integrateGPU (TSDFVolume.cpp:148-346, TSDFVolume.h:89-96,
  * 169-176): d_depth_m is W*H float metres on the device. */

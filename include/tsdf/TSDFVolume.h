@@ -161,6 +161,30 @@ public:
         for (size_t i = 0; i < n; ++i) out[i] = Voxel{t[i], w[i]};
         return out;
     }
+    // TSDFVolume::extractGlobalPointCloud (TSDFVolume.cpp:571-632): the voxels with
+    // weight > 1 and |tsdf| < 0.2 at origin + (x, y, z) * voxel_size, in linear voxel
+    // order, computed on the GPU. Vec3 is any type built from three floats
+    // (Eigen::Vector3f, std::array<float, 3>); colours are zeros (colour is not stored).
+    template <class Vec3>
+    void extractGlobalPointCloud(std::vector<Vec3>& points_out, std::vector<uint8_t>& colors_out) const {
+        unsigned long long n = 0;
+        detail::check(azu_vol_point_cloud(h_, nullptr, 0, &n, stream_), "TSDFVolume::extractGlobalPointCloud");
+        std::vector<float> xyz(3 * static_cast<size_t>(n));
+        if (n > 0) {
+            detail::DeviceBuffer d(static_cast<size_t>(n) * 12);
+            unsigned long long m = 0;
+            detail::check(azu_vol_point_cloud(h_, static_cast<float*>(d.p), n, &m, stream_),
+                          "TSDFVolume::extractGlobalPointCloud");
+            detail::check(azu_memcpy_d2h(xyz.data(), d.p, static_cast<size_t>(n) * 12, stream_),
+                          "TSDFVolume::extractGlobalPointCloud download");
+            detail::check(azu_stream_sync(stream_), "TSDFVolume::extractGlobalPointCloud");
+        }
+        points_out.clear();
+        points_out.reserve(static_cast<size_t>(n));
+        for (size_t i = 0; i < static_cast<size_t>(n); ++i)
+            points_out.push_back(Vec3{xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]});
+        colors_out.assign(3 * static_cast<size_t>(n), 0);
+    }
     Voxel voxelAt(int x, int y, int z) const {
         const size_t r = static_cast<size_t>(params_.resolution);
         return voxelData()[(static_cast<size_t>(z) * r + y) * r + x];

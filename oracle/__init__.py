@@ -235,3 +235,20 @@ def mc_extract(tsdf, weight, voxel_size, origin, max_tris=2000000):
     lib.orc_mc_extract(_d(t), _d(w), res, voxel_size, _d(o), max_tris, _d(pos), _d(nrm), _d(idx), ctypes.byref(nv),
                        ctypes.byref(ni), ctypes.byref(tr))
     return pos, nrm, idx, bool(tr.value)
+
+
+def point_cloud(tsdf, weight, voxel_size, origin):
+    """TSDFVolume::extractGlobalPointCloud (src/tsdf/TSDFVolume.cpp:571-632) restated
+    serially: every voxel with weight > 1 and |tsdf| < 0.2, at voxelToWorld(x, y, z) =
+    origin + (x, y, z) * voxel_size (TSDFVolume.cpp:559-561) in float32, in linear voxel
+    order (z, y, x with x fastest; the reference's CPU form concatenates per-thread lists,
+    its GPU form compacts in this order). tsdf / weight are (res, res, res) arrays indexed
+    [z, y, x]."""
+    t = np.asarray(tsdf, np.float32)
+    w = np.asarray(weight, np.float32)
+    keep = (w > np.float32(1.0)) & (np.abs(t) < np.float32(0.2))
+    z, y, x = np.nonzero(keep)
+    vs = np.float32(voxel_size)
+    o = np.asarray(origin, np.float32)
+    return np.stack([o[0] + x.astype(np.float32) * vs, o[1] + y.astype(np.float32) * vs,
+                     o[2] + z.astype(np.float32) * vs], axis=1).astype(np.float32)

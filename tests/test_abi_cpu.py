@@ -69,6 +69,7 @@ CPP_LOOP = r"""
 #include "tracking/ICPTracker.h"
 #include "tracking/TrackingPolicy.h"
 #include "meshing/MarchingCubes.h"
+#include <array>
 #include <cstdio>
 struct Mat4 { float m[16]; const float* data() const { return m; } };
 int run(const float* d_depth, int W, int H, int frames) {
@@ -98,6 +99,9 @@ int run(const float* d_depth, int W, int H, int frames) {
     }
     meshing::MarchingCubes mc;
     auto mesh = mc.extract(vol, [](float x) { std::printf("%f\n", x); });
+    std::vector<std::array<float, 3>> cloud;
+    std::vector<uint8_t> colors;
+    vol.extractGlobalPointCloud(cloud, colors);
     return static_cast<int>(mesh->triangleCount());
 }
 int main(int argc, char**) { return argc > 99 ? run(nullptr, 640, 480, 2) : 0; }

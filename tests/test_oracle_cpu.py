@@ -194,3 +194,43 @@ def test_floor_to_int_and_scale_intrinsics():
     assert o.tolist() == list(K)
     oracle.lib.orc_scale_intrinsics(K.ctypes.data, 640, 480, 1280, 960, o.ctypes.data)
     assert o.tolist() == [1050.0, 1050.0, 639.5, 479.5]
+
+
+def planar_field(res, vs, trunc, z0):
+    """A planar TSDF written on paper: voxel (x, y, z) at depth origin_z + z * vs holds
+    clip((z0 - (origin_z + z * vs)) / trunc, -1, 1) with weight 2, the rest weight 0."""
+    zs = np.arange(res, dtype=np.float64) * vs
+    t = np.clip((z0 - zs) / trunc, -1.0, 1.0).astype(np.float32)
+    tsdf = np.broadcast_to(t[:, None, None], (res, res, res)).copy()
+    weight = np.full((res, res, res), 2.0, np.float32)
+    return tsdf, weight
+
+
+def test_point_cloud_oracle_closed_forms():
+    """The point cloud of a planar field is whole x-y layers: the layers whose |tsdf| is
+    below 0.2, i.e. |z0 - z vs| < 0.2 trunc, so res^2 points per layer; the predicate is
+    strict on both bounds (weight exactly 1 and |tsdf| exactly 0.2 are out), and NaN
+    fails it."""
+    res, vs, trunc = 16, 0.01, 0.05
+    origin = (-0.08, -0.08, 0.0)
+    tsdf, weight = planar_field(res, vs, trunc, z0=0.075)
+    pts = oracle.point_cloud(tsdf, weight, vs, origin)
+    layers = [z for z in range(res) if abs(0.075 - z * vs) < 0.2 * trunc - 1e-9]
+    assert layers == [7, 8]
+    assert pts.shape == (len(layers) * res * res, 3)
+    # linear order: z, then y, then x fastest
+    assert np.all(pts[:res, 0] == np.float32(-0.08) + np.arange(res, dtype=np.float32) * np.float32(vs))
+    assert np.all(pts[:, 2][: res * res] == np.float32(7) * np.float32(vs))
+    # strict bounds and NaN
+    t = np.zeros((2, 2, 2), np.float32)
+    w = np.full((2, 2, 2), 2.0, np.float32)
+    w[0, 0, 0] = 1.0                      # weight exactly 1: out
+    w[0, 0, 1] = np.nextafter(np.float32(1), np.float32(2))  # just above: in
+    t[0, 1, 0] = np.float32(0.2)          # |tsdf| exactly 0.2: out
+    t[0, 1, 1] = np.float32(-0.2)         # out
+    t[1, 0, 0] = np.nextafter(np.float32(0.2), np.float32(0))  # in
+    t[1, 0, 1] = np.nan                   # out
+    w[1, 1, 0] = np.nan                   # out
+    pts = oracle.point_cloud(t, w, 1.0, (0, 0, 0))
+    # (x, y, z) of the kept voxels, in linear order
+    assert [tuple(int(v) for v in p) for p in pts] == [(1, 0, 0), (0, 0, 1), (1, 1, 1)]
