@@ -12,6 +12,9 @@
 #include <memory>
 #include <vector>
 
+#include <cmath>
+#include <string>
+
 #include "../azu_hip.h"
 #include "../tsdf/TSDFVolume.h"
 
@@ -35,6 +38,42 @@ struct MeshData {
     bool empty() const { return positions.empty(); }
     size_t triangleCount() const { return indices.size() / 3; }
     bool hasColors() const { return !colors.empty(); }
+
+    // MeshData::validate (reference include/meshing/MeshData.h:49-97): the mesh can be
+    // indexed and serialised without out-of-bounds reads or non-finite coordinates.
+    // Checks in this order and reports the first failure through `reason` (may be
+    // null): positions empty, indices empty, index count not a multiple of 3, an index
+    // out of range, a non-finite position, a normal count neither 0 nor the vertex
+    // count, a non-finite normal, a colour count neither 0 nor 3 per vertex.
+    bool validate(std::string* reason) const {
+        auto reject = [&](const std::string& why) {
+            if (reason) *reason = why;
+            return false;
+        };
+        auto finite3 = [](const std::array<float, 3>& v) {
+            return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]);
+        };
+        const size_t nv = positions.size(), ni = indices.size();
+        if (nv == 0) return reject("positions is empty");
+        if (ni == 0) return reject("indices is empty while positions.size()=" + std::to_string(nv));
+        if (ni % 3 != 0) return reject("indices.size()=" + std::to_string(ni) + " is not a multiple of 3");
+        for (size_t i = 0; i < ni; ++i)
+            if (static_cast<size_t>(indices[i]) >= nv)
+                return reject("indices[" + std::to_string(i) + "]=" + std::to_string(indices[i]) +
+                              " is out of range for positions.size()=" + std::to_string(nv));
+        for (size_t i = 0; i < nv; ++i)
+            if (!finite3(positions[i])) return reject("positions[" + std::to_string(i) + "] has a non-finite component");
+        if (!normals.empty() && normals.size() != nv)
+            return reject("normals.size()=" + std::to_string(normals.size()) + " is neither 0 nor positions.size()=" +
+                          std::to_string(nv));
+        for (size_t i = 0; i < normals.size(); ++i)
+            if (!finite3(normals[i])) return reject("normals[" + std::to_string(i) + "] has a non-finite component");
+        const size_t nc = colors.size();
+        if (nc != 0 && (nc % 3 != 0 || nc / 3 != nv))
+            return reject("colors.size()=" + std::to_string(nc) + " is neither 0 nor 3 * positions.size() (" +
+                          std::to_string(nv) + " vertices)");
+        return true;
+    }
 };
 
 using ProgressCallback = std::function<void(float)>;

@@ -69,6 +69,7 @@ CPP_LOOP = r"""
 #include "tracking/ICPTracker.h"
 #include "tracking/TrackingPolicy.h"
 #include "meshing/MarchingCubes.h"
+#include "export/PLYExporter.h"
 #include <array>
 #include <cstdio>
 struct Mat4 { float m[16]; const float* data() const { return m; } };
@@ -102,6 +103,7 @@ int run(const float* d_depth, int W, int H, int frames) {
     std::vector<std::array<float, 3>> cloud;
     std::vector<uint8_t> colors;
     vol.extractGlobalPointCloud(cloud, colors);
+    export_io::PLYExporter::writeBinary(*mesh, "mesh.ply");
     return static_cast<int>(mesh->triangleCount());
 }
 int main(int argc, char**) { return argc > 99 ? run(nullptr, 640, 480, 2) : 0; }
