@@ -29,7 +29,7 @@ hipError_t launchRaycast(int format, bool exact, const void* vox, const VolParam
                          const unsigned char* bricks, unsigned* dsafe, int* ok, const PoseArg& pa, Intr K, int W,
                          int H, float* verts, float* norms, float* t_hit, hipStream_t st);
 hipError_t launchSkipInit(unsigned* dsafe, size_t cells, hipStream_t st);
-hipError_t launchLive(bool mm, const void* src, int W, int H, Intr K, float min_d, float max_d, float* const d[3],
+hipError_t launchLive(int src_kind, const void* src, int W, int H, Intr K, float min_d, float max_d, float* const d[3],
                       float* const v[3], float* const n[3], hipStream_t st);
 size_t icpStateBytes();
 int icpBlocksFor(int W, int H);
@@ -778,16 +778,17 @@ int azu_tracker_set_params(azu_tracker* t, const azu_icp_params* p) {
     return AZU_OK;
 }
 
-static int prepareImpl(azu_tracker* t, const void* src, bool mm, azu_stream s) {
+static int prepareImpl(azu_tracker* t, const void* src, int src_kind, azu_stream s) {
     if (!t || !src) return fail(AZU_ERR_ARG, "null tracker or depth");
     ProfScope ps("live_pyramid", (hipStream_t)s);
-    AZU_HIP_TRY(launchLive(mm, src, t->W, t->H, t->K, t->params.min_depth, t->params.max_depth, t->d, t->v, t->n,
+    AZU_HIP_TRY(launchLive(src_kind, src, t->W, t->H, t->K, t->params.min_depth, t->params.max_depth, t->d, t->v, t->n,
                            (hipStream_t)s),
                 "live pyramid launch");
     return AZU_OK;
 }
-int azu_tracker_prepare_mm(azu_tracker* t, const uint16_t* d, azu_stream s) { return prepareImpl(t, d, true, s); }
-int azu_tracker_prepare_m(azu_tracker* t, const float* d, azu_stream s) { return prepareImpl(t, d, false, s); }
+int azu_tracker_prepare_mm(azu_tracker* t, const uint16_t* d, azu_stream s) { return prepareImpl(t, d, 1, s); }
+int azu_tracker_prepare_m(azu_tracker* t, const float* d, azu_stream s) { return prepareImpl(t, d, 0, s); }
+int azu_tracker_prepare_raw(azu_tracker* t, const uint16_t* d, azu_stream s) { return prepareImpl(t, d, 2, s); }
 
 int azu_tracker_load_level(azu_tracker* t, int level, const float* d_depth, const float* d_vertices,
                            const float* d_normals, azu_stream s) {

@@ -1,7 +1,7 @@
 // Live frame geometry on gfx950: the reference's buildFrameData / updateVerticesFromDepth
 // / computeNormals / buildFramePyramid (src/sensor/FrameData.cpp:113-345).
 //
-//  * liveLevel0: per pixel, millimetres (or metres) -> band-gated metres, back-projected
+//  * liveLevel0: per pixel, millimetres, metres or raw Kinect codes -> band-gated metres, back-projected
 //    vertex, and the 4-neighbour cross-product normal with the depth-jump guard. The
 //    neighbours' vertices are recomputed from their depths (same formula, same bits),
 //    so one kernel produces the whole level.
@@ -16,11 +16,21 @@ namespace azu {
 
 __device__ __forceinline__ float depthJump(float d) { return smax(0.03f, d * 0.05f); }
 
-template <bool MM>
+// Source encodings of the live depth: float metres, uint16 millimetres, or uint16 raw
+// Kinect v1 11-bit codes (DepthValidity.h:47-93: raw 0 and raw >= 2047 are invalid,
+// metres = 1 / (raw * -0.0030711016 + 3.3309495161), which has its pole at raw ~1084.6
+// and is negative beyond it; the band gate rejects both).
+enum LiveSrc { kSrcMetres = 0, kSrcMm = 1, kSrcRaw = 2 };
+
+template <int SRC>
 __device__ __forceinline__ float liveDepth(const void* src, int i, float min_d, float max_d) {
     float m;
     bool nz;
-    if (MM) {
+    if (SRC == kSrcRaw) {
+        const uint16_t v = static_cast<const uint16_t*>(src)[i];
+        nz = v != 0 && v < 2047;
+        m = nz ? 1.0f / ((float)v * -0.0030711016f + 3.3309495161f) : 0.0f;
+    } else if (SRC == kSrcMm) {
         const uint16_t v = static_cast<const uint16_t*>(src)[i];
         m = (float)v * 0.001f;
         nz = v != 0;
@@ -43,7 +53,7 @@ __device__ __forceinline__ void st3(float* p, int i, F3 v) {
 }
 __device__ __forceinline__ F3 ld3(const float* p, int i) { return F3{p[3 * i + 0], p[3 * i + 1], p[3 * i + 2]}; }
 
-template <bool MM>
+template <int SRC>
 __global__ __launch_bounds__(256) void liveLevel0(const void* __restrict__ src, int W, int H, Intr K, float min_d,
                                                   float max_d, float* __restrict__ d0, float* __restrict__ v0,
                                                   float* __restrict__ n0) {
@@ -52,15 +62,15 @@ __global__ __launch_bounds__(256) void liveLevel0(const void* __restrict__ src, 
     if (x >= W || y >= H) return;
     const int c = y * W + x;
     const float fx_inv = 1.0f / K.fx, fy_inv = 1.0f / K.fy;
-    const float dc = liveDepth<MM>(src, c, min_d, max_d);
+    const float dc = liveDepth<SRC>(src, c, min_d, max_d);
     d0[c] = dc;
     st3(v0, c, backProject(x, y, dc, fx_inv, fy_inv, K));
     F3 n{0, 0, 0};
     if (x >= 1 && x < W - 1 && y >= 1 && y < H - 1) {
-        const float dr = liveDepth<MM>(src, c + 1, min_d, max_d);
-        const float dl = liveDepth<MM>(src, c - 1, min_d, max_d);
-        const float du = liveDepth<MM>(src, c - W, min_d, max_d);
-        const float dd = liveDepth<MM>(src, c + W, min_d, max_d);
+        const float dr = liveDepth<SRC>(src, c + 1, min_d, max_d);
+        const float dl = liveDepth<SRC>(src, c - 1, min_d, max_d);
+        const float du = liveDepth<SRC>(src, c - W, min_d, max_d);
+        const float dd = liveDepth<SRC>(src, c + W, min_d, max_d);
         if (dc > 0.0f && dr > 0.0f && dl > 0.0f && du > 0.0f && dd > 0.0f) {
             const float jump = depthJump(dc);
             if (!(fabsf(dc - dr) > jump || fabsf(dc - dl) > jump || fabsf(dc - du) > jump || fabsf(dc - dd) > jump)) {
@@ -172,13 +182,16 @@ __global__ __launch_bounds__(256) void livePyramid(int W0, int H0, const float* 
     }
 }
 
-hipError_t launchLive(bool mm, const void* src, int W, int H, Intr K, float min_d, float max_d, float* const d[3],
+hipError_t launchLive(int src_kind, const void* src, int W, int H, Intr K, float min_d, float max_d, float* const d[3],
                       float* const v[3], float* const n[3], hipStream_t st) {
     const dim3 g0((W + 15) / 16, (H + 15) / 16);
-    if (mm)
-        hipLaunchKernelGGL(liveLevel0<true>, g0, dim3(256), 0, st, src, W, H, K, min_d, max_d, d[0], v[0], n[0]);
+    if (src_kind == kSrcMm)
+        hipLaunchKernelGGL(liveLevel0<kSrcMm>, g0, dim3(256), 0, st, src, W, H, K, min_d, max_d, d[0], v[0], n[0]);
+    else if (src_kind == kSrcRaw)
+        hipLaunchKernelGGL(liveLevel0<kSrcRaw>, g0, dim3(256), 0, st, src, W, H, K, min_d, max_d, d[0], v[0], n[0]);
     else
-        hipLaunchKernelGGL(liveLevel0<false>, g0, dim3(256), 0, st, src, W, H, K, min_d, max_d, d[0], v[0], n[0]);
+        hipLaunchKernelGGL(liveLevel0<kSrcMetres>, g0, dim3(256), 0, st, src, W, H, K, min_d, max_d, d[0], v[0],
+                           n[0]);
     const int W1 = W / 2, H1 = H / 2;
     const dim3 g1(((W1 + 1) / 2 + 15) / 16, ((H1 + 1) / 2 + 3) / 4);
     hipLaunchKernelGGL(livePyramid, g1, dim3(256), 0, st, W, H, d[0], v[0], n[0], d[1], v[1], n[1], d[2], v[2],

@@ -237,6 +237,10 @@ class ICPTracker:
     def prepare_mm_device(self, d_mm, stream=None):
         check(lib.azu_tracker_prepare_mm(self.h, d_mm, stream))
 
+    def prepare_raw_device(self, d_raw, stream=None):
+        """Live pyramid from device uint16 raw Kinect codes (azu_tracker_prepare_raw)."""
+        check(lib.azu_tracker_prepare_raw(self.h, d_raw, stream))
+
     def prepare_m_device(self, d_m, stream=None):
         check(lib.azu_tracker_prepare_m(self.h, d_m, stream))
 

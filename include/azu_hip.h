@@ -236,6 +236,12 @@ int azu_tracker_set_params(azu_tracker* t, const azu_icp_params* p);
  * uint16 millimetres (x 0.001 m) or float metres, band-gated to [min_depth, max_depth]. */
 int azu_tracker_prepare_mm(azu_tracker* t, const uint16_t* d_depth_mm, azu_stream s);
 int azu_tracker_prepare_m(azu_tracker* t, const float* d_depth_m, azu_stream s);
+/* Same from raw Kinect v1 11-bit codes (the reference's sensor boundary, cpuDepthMeters,
+ * include/sensor/DepthValidity.h:55-93, feeding buildFrameData, FrameData.cpp:137-194):
+ * raw 0 and raw >= 2047 are invalid, metres = 1 / (raw * -0.0030711016 + 3.3309495161)
+ * in float, then the same [min_depth, max_depth] band (which also rejects the codes
+ * around and beyond the curve's pole at raw ~1084.6). */
+int azu_tracker_prepare_raw(azu_tracker* t, const uint16_t* d_raw, azu_stream s);
 /* ICPTracker::track takes a host FramePyramid (ICPTracker.h:91-94); these load one
  * level of it from device buffers instead (copies; d_depth may be NULL for levels the
  * ICP reads only as vertices / normals), and rebuild levels 1 and 2 from level 0 with

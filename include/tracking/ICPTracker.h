@@ -116,6 +116,12 @@ public:
         ensure(width, height);
         detail::check(azu_tracker_prepare_mm(h_, d_depth_mm, stream_), "ICPTracker::prepareLiveGPU");
     }
+    // The same from device raw Kinect v1 11-bit codes (the reference's sensor
+    // boundary, DepthValidity.h:55-93: invalid codes and out-of-band metres -> 0).
+    void prepareLiveRawGPU(const uint16_t* d_raw, int width, int height) {
+        ensure(width, height);
+        detail::check(azu_tracker_prepare_raw(h_, d_raw, stream_), "ICPTracker::prepareLiveRawGPU");
+    }
     // Host depth in metres (uploaded, then prepareLiveGPU).
     void prepareLive(const float* depth_m, int width, int height) {
         const size_t bytes = static_cast<size_t>(width) * height * sizeof(float);

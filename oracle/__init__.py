@@ -252,3 +252,15 @@ def point_cloud(tsdf, weight, voxel_size, origin):
     o = np.asarray(origin, np.float32)
     return np.stack([o[0] + x.astype(np.float32) * vs, o[1] + y.astype(np.float32) * vs,
                      o[2] + z.astype(np.float32) * vs], axis=1).astype(np.float32)
+
+
+def depth_raw(raw, min_d=0.3, max_d=5.0):
+    """cpuDepthMeters (include/sensor/DepthValidity.h:55-93) restated in float32: raw 0
+    and raw >= 2047 -> 0; else 1 / (raw * -0.0030711016 + 3.3309495161), kept when finite
+    and inside [min_d, max_d], else 0 (no clamping)."""
+    r = np.asarray(raw, np.uint16)
+    valid = (r != 0) & (r < 2047)
+    with np.errstate(divide="ignore", over="ignore", invalid="ignore"):
+        m = np.float32(1.0) / (r.astype(np.float32) * np.float32(-0.0030711016) + np.float32(3.3309495161))
+    keep = valid & np.isfinite(m) & (m >= np.float32(min_d)) & (m <= np.float32(max_d))
+    return np.where(keep, m, np.float32(0.0)).astype(np.float32)

@@ -234,3 +234,25 @@ def test_point_cloud_oracle_closed_forms():
     pts = oracle.point_cloud(t, w, 1.0, (0, 0, 0))
     # (x, y, z) of the kept voxels, in linear order
     assert [tuple(int(v) for v in p) for p in pts] == [(1, 0, 0), (0, 0, 1), (1, 1, 1)]
+
+
+def test_depth_raw_oracle_golden_facts():
+    """tests/depth_domain_contract.cpp:140-230 and tests/cpu_smoke_harness.cpp:48-57 of the
+    reference: the predicate over all 65536 codes, invalid -> exactly +0.0, the default
+    band [0.3, 2.5] accepts exactly raw 1..954, the pole codes 1084 / 1085 and 1500 / 2046
+    are rejected, nothing negative / NaN / Inf comes out, raw 700 lands in the band, and
+    every accepted value matches the double-precision curve."""
+    raw = np.arange(65536, dtype=np.uint16)
+    m = oracle.depth_raw(raw, 0.3, 2.5)
+    assert np.all(np.isfinite(m)) and np.all(m >= 0)
+    zero = m == 0
+    assert np.all(np.signbit(m[zero]) == False)  # noqa: E712  (+0.0, not -0.0)
+    accepted = np.nonzero(~zero)[0]
+    assert accepted[0] == 1 and accepted[-1] == 954 and len(accepted) == 954
+    for r in (0, 955, 1084, 1085, 1500, 2046, 2047, 2048, 65535):
+        assert m[r] == 0.0, r
+    d = 1.0 / (accepted.astype(np.float64) * np.float64(np.float32(-0.0030711016)) +
+               np.float64(np.float32(3.3309495161)))
+    assert np.max(np.abs(m[accepted] - d)) < 1e-6
+    assert 0.3 < oracle.depth_raw(np.uint16(700), 0.3, 2.5) < 2.5
+    assert oracle.depth_raw(np.uint16(500), 0.3, 5.0) < oracle.depth_raw(np.uint16(700), 0.3, 5.0)
