@@ -142,11 +142,14 @@ class GradedFrameLoop(FrameLoop):
         check(lib.azu_track_grade(ctypes.byref(r), self._prev, ctypes.byref(self.policy), rel, ctypes.byref(d)))
         return d
 
-    def frame(self, d_mm):
-        """Run one frame from device uint16 millimetres; returns its decision record."""
+    def frame(self, d_src, raw=False):
+        """Run one frame from device uint16 millimetres (raw=True: raw Kinect codes);
+        returns its decision record (quality, degenerate_dofs, retried, inliers,
+        weakest_ratio, result: the graded IcpResult, predicted: the estimate's pose)."""
         s = self.stream
-        check(lib.azu_tracker_prepare_mm(self.tr.h, d_mm, s))
-        rec = dict(quality=N.TRACK_GOOD, degenerate_dofs=0, retried=False, inliers=0, weakest_ratio=None)
+        check((lib.azu_tracker_prepare_raw if raw else lib.azu_tracker_prepare_mm)(self.tr.h, d_src, s))
+        rec = dict(quality=N.TRACK_GOOD, degenerate_dofs=0, retried=False, inliers=0, weakest_ratio=None,
+                   result=None, predicted=None)
         integrate = True
         if self.nframes > 0:
             check(lib.azu_memcpy_d2d(self.d_prev.ptr, self.d_pose, 64, s))
@@ -155,7 +158,10 @@ class GradedFrameLoop(FrameLoop):
                 pred = ctypes.c_void_p()
                 check(lib.azu_tracker_predict(self.tr.h, ctypes.byref(pred), s))
                 est, predicted = pred.value, True
+            pose_est = np.empty(16, np.float32)
+            check(lib.azu_memcpy_d2h(pose_est.ctypes.data, est, 64, s))
             r = self._track(est)
+            rec["predicted"] = pose_rm(pose_est)
             if predicted and self._grade(r, 0.0).quality == N.TRACK_FAILED:
                 r = self._track(self.d_prev.ptr)
                 rec["retried"] = True
@@ -164,8 +170,9 @@ class GradedFrameLoop(FrameLoop):
             self._prev = (ctypes.c_float * 16)(*d.pose)
             ratio = ctypes.c_float()
             check(lib.azu_track_weakest_ratio(r.information, ctypes.byref(ratio)))
+            r.pose[:] = d.pose
             rec.update(quality=d.quality, degenerate_dofs=d.degenerate_dofs, inliers=r.inliers,
-                       weakest_ratio=ratio.value)
+                       weakest_ratio=ratio.value, result=r)
             integrate = bool(d.integrate)
             if d.quality == N.TRACK_FAILED:
                 self.consecutive_failures += 1
