@@ -373,10 +373,16 @@ __global__ __launch_bounds__(256) void pointCount(McVol<S> V, uint32_t* __restri
     if (row >= (size_t)V.R * V.R) return;
     const typename S::Word* src = V.vox + row * V.R;
     uint32_t n = 0;
-    for (int x0 = 0; x0 < V.R; x0 += 64) {
-        const int x = x0 + lane;
-        const bool keep = x < V.R && pcKeep<S>(src[x]);
-        n += (uint32_t)__popcll(__ballot(keep));
+    // four 64-voxel chunks per step, loads issued before the ballots
+    for (int x0 = 0; x0 < V.R; x0 += 256) {
+        typename S::Word w[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int x = x0 + 64 * k + lane;
+            w[k] = x < V.R ? src[x] : S::empty();
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) n += (uint32_t)__popcll(__ballot(pcKeep<S>(w[k])));
     }
     if (lane == 0) row_count[row] = n;
 }
@@ -391,18 +397,28 @@ __global__ __launch_bounds__(256) void pointWrite(McVol<S> V, const uint32_t* __
     const float py = V.oy + (float)y * V.vs, pz = V.oz + (float)z * V.vs;
     const typename S::Word* src = V.vox + row * V.R;
     unsigned long long base = row_off[row];
-    for (int x0 = 0; x0 < V.R; x0 += 64) {
-        const int x = x0 + lane;
-        const bool keep = x < V.R && pcKeep<S>(src[x]);
-        const unsigned long long mask = __ballot(keep);
-        const unsigned rank = __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
-        const unsigned long long k = base + rank;
-        if (keep && k < capacity) {
-            out[3 * k + 0] = V.ox + (float)x * V.vs;
-            out[3 * k + 1] = py;
-            out[3 * k + 2] = pz;
+    for (int x0 = 0; x0 < V.R; x0 += 256) {
+        typename S::Word w[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int x = x0 + 64 * c + lane;
+            w[c] = x < V.R ? src[x] : S::empty();
         }
-        base += (unsigned long long)__popcll(mask);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int x = x0 + 64 * c + lane;
+            const bool keep = pcKeep<S>(w[c]);
+            const unsigned long long mask = __ballot(keep);
+            const unsigned rank =
+                __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+            const unsigned long long k = base + rank;
+            if (keep && k < capacity) {
+                out[3 * k + 0] = V.ox + (float)x * V.vs;
+                out[3 * k + 1] = py;
+                out[3 * k + 2] = pz;
+            }
+            base += (unsigned long long)__popcll(mask);
+        }
     }
 }
 

@@ -79,3 +79,26 @@ def test_point_cloud_capacity_below_count():
     assert got_n.value == n == len(want)
     assert np.array_equal(out[:cap], want[:cap])
     assert np.all(out[cap:] == -7.0)
+
+
+@pytest.mark.parametrize("fmt", [N.FMT_F32, N.FMT_F16_U16])
+def test_point_cloud_fused_512(fmt):
+    """The 512^3 / 2 mm volume after 4 frames of the sphere + plane + 10 boxes sequence:
+    bit-exact against the oracle at full size, and timed (both passes, the scan and the
+    count read-back; event-timed over 3 calls)."""
+    import time
+    frames, truth, K = sc.sequence("c2_512", frames=4)
+    vol = gpu_volume(sc.volume_params("c2_512"), fmt=fmt)
+    for f, T in zip(frames, truth):
+        vol.integrate(f.astype(np.float32) * np.float32(0.001), K, T)
+    got = vol.point_cloud()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        vol.point_cloud()
+    ms = (time.perf_counter() - t0) / 3 * 1e3
+    want = cloud_of(vol)
+    vol.close()
+    say("point cloud 512^3 fmt %d: %d points, %.2f ms per extraction (host clock, incl. download)"
+        % (fmt, len(want), ms))
+    assert len(want) > 100000
+    assert np.array_equal(got, want)
