@@ -296,8 +296,9 @@ inline ObservableMotion keepObservableMotion(const float* prev, const float* est
 
 // The same calls for matrix types with data() giving the column-major array
 // (Eigen::Matrix4f, Eigen::Matrix<float, 6, 6>), as the reference's signatures take them.
-template <class M, class = decltype(std::declval<const M&>().data())>
-inline bool withinFrameMotion(const M& pose, const M& prev_pose, const TrackingPolicy& p = TrackingPolicy{}) {
+template <class A, class B, class = decltype(std::declval<const A&>().data()),
+          class = decltype(std::declval<const B&>().data())>
+inline bool withinFrameMotion(const A& pose, const B& prev_pose, const TrackingPolicy& p = TrackingPolicy{}) {
     return withinFrameMotion(pose.data(), prev_pose.data(), p);
 }
 template <class M, class = decltype(std::declval<const M&>().data())>
@@ -308,9 +309,9 @@ template <class M, class = decltype(std::declval<const M&>().data())>
 inline float weakestDirectionRatio(const M& information) {
     return weakestDirectionRatio(information.data());
 }
-template <class M4, class M6, class = decltype(std::declval<const M4&>().data()),
-          class = decltype(std::declval<const M6&>().data())>
-inline ObservableMotion keepObservableMotion(const M4& prev, const M4& est, const M6& information,
+template <class P, class E, class M6, class = decltype(std::declval<const P&>().data()),
+          class = decltype(std::declval<const E&>().data()), class = decltype(std::declval<const M6&>().data())>
+inline ObservableMotion keepObservableMotion(const P& prev, const E& est, const M6& information,
                                              float rel_threshold = 5e-3f) {
     return keepObservableMotion(prev.data(), est.data(), information.data(), rel_threshold);
 }
