@@ -50,3 +50,22 @@ def test_mc_empty_and_tiny():
     v = TSDFVolume(TSDFParams(2, 0.01, 0.03, 128.0, (0, 0, 0), 0.3, 5.0))
     pos, nrm, idx, tr = v.marching_cubes()
     assert len(pos) == 0 and len(idx) == 0 and not tr
+
+
+@pytest.mark.parametrize("fmt", [N.FMT_F32, N.FMT_F16_U16])
+def test_mc_bitwise_512(fmt):
+    """The 512^3 / 2 mm volume after 4 frames of the sphere + plane + 10 boxes sequence:
+    bit-exact against the CPU mesher at full size, and timed (host clock over 3
+    extractions, allocation and download included)."""
+    import time
+    frames_, truth, K = sc.sequence("c2_512", frames=4)
+    v = TSDFVolume(TSDFParams(**sc.volume_params("c2_512")), fmt=fmt)
+    for mm, T in zip(frames_, truth):
+        v.integrate(mm.astype(np.float32) * np.float32(0.001), K, T)
+    n = _check(v)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        v.marching_cubes()
+    say("mesh 512^3 fmt %d: %d triangles, %.2f ms per extraction" % (fmt, n, (time.perf_counter() - t0) / 3 * 1e3))
+    v.close()
+    assert n > 100000
