@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <mutex>
 #include <string>
@@ -189,7 +190,7 @@ bool finiteK(const float* K) {
            K[0] != 0.0f && K[1] != 0.0f;
 }
 
-// scaleIntrinsics (include/sensor/CameraIntrinsics.h:85-92)
+// scaleIntrinsics (include/sensor/CameraIntrinsics.h:33-40)
 Intr scaleIntrinsics(Intr k, int rw, int rh, int w, int h) {
     if (w == rw && h == rh) return k;
     const float sx = (float)w / (float)rw, sy = (float)h / (float)rh;
@@ -775,6 +776,26 @@ int azu_tracker_create(int W, int H, const float K[4], const azu_icp_params* p, 
 int azu_tracker_set_params(azu_tracker* t, const azu_icp_params* p) {
     if (!t || !p) return fail(AZU_ERR_ARG, "null argument");
     t->params = sanitize(*p);
+    return AZU_OK;
+}
+
+int azu_tracker_get_params(const azu_tracker* t, azu_icp_params* out) {
+    if (!t || !out) return fail(AZU_ERR_ARG, "null argument");
+    *out = t->params;
+    return AZU_OK;
+}
+
+int azu_icp_params_sanitize(const azu_icp_params* in, azu_icp_params* out) {
+    if (!in || !out) return fail(AZU_ERR_ARG, "null argument");
+    *out = sanitize(*in);
+    return AZU_OK;
+}
+
+int azu_icp_result_default(azu_icp_result* out) {
+    if (!out) return fail(AZU_ERR_ARG, "null argument");
+    std::memset(out, 0, sizeof(*out));
+    for (int i = 0; i < 4; ++i) out->pose[i * 5] = 1.0f;
+    out->final_step = std::numeric_limits<float>::infinity();
     return AZU_OK;
 }
 

@@ -186,7 +186,7 @@ __device__ void jacobiMinMax(const double* Ain, double& mn, double& mx) {
     }
 }
 
-// dampingForHessian (ICPShared.h:293-307): lambda is the level's base (0.01 at level
+// dampingForHessian (ICPShared.h:86-100): lambda is the level's base (0.01 at level
 // 0, 0.1 above) unless min_eig < 1e-4 or max_eig > 1e7 min_eig, then 1.0. A Cholesky
 // certificate settles the common case without an eigen-solve: with g >= max_eig
 // (Gershgorin) and s = max(1e-4, g / 1e7), a successful factorisation of A - s I
@@ -250,7 +250,7 @@ __device__ void solveLane(const double* sums, int level, int lane, SolveLanes* o
 }
 
 // Nearest rotation by one-sided Jacobi SVD in FP64, U V^T with U's smallest-singular-
-// value column negated if det < 0 (projectToSO3, ICPShared.h:309-317). General fallback.
+// value column negated if det < 0 (projectToSO3, ICPShared.h:102-110). General fallback.
 __device__ void projectSO3Svd(const double* M, double* R) {
     double U[9], V[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
     for (int i = 0; i < 9; ++i) U[i] = M[i];

@@ -141,8 +141,11 @@ _SIGS = {
                                           ctypes.POINTER(c_void_p)]),
     "azu_tracker_destroy": (ctypes.c_int, [c_void_p]),
     "azu_tracker_set_params": (ctypes.c_int, [c_void_p, ctypes.POINTER(IcpParams)]),
+    "azu_tracker_get_params": (ctypes.c_int, [c_void_p, ctypes.POINTER(IcpParams)]),
+    "azu_icp_params_sanitize": (ctypes.c_int, [ctypes.POINTER(IcpParams), ctypes.POINTER(IcpParams)]),
+    "azu_icp_result_default": (ctypes.c_int, [ctypes.POINTER(IcpResult)]),
     "azu_tracker_prepare_mm": (ctypes.c_int, [c_void_p, c_void_p, c_void_p]),
-"azu_tracker_prepare_raw": (ctypes.c_int, [c_void_p, c_void_p, c_void_p]),
+    "azu_tracker_prepare_raw": (ctypes.c_int, [c_void_p, c_void_p, c_void_p]),
     "azu_tracker_prepare_m": (ctypes.c_int, [c_void_p, c_void_p, c_void_p]),
     "azu_tracker_load_level": (ctypes.c_int, [c_void_p, ctypes.c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "azu_tracker_downsample": (ctypes.c_int, [c_void_p, c_void_p]),

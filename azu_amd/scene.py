@@ -71,7 +71,7 @@ def upsample2x(mm):
 
 
 def scale_intrinsics(K, rw, rh, w, h):
-    """scaleIntrinsics (include/sensor/CameraIntrinsics.h:85-92), in float32."""
+    """scaleIntrinsics (include/sensor/CameraIntrinsics.h:33-40), in float32."""
     K = np.asarray(K, np.float32)
     if w == rw and h == rh:
         return K.copy()

@@ -95,7 +95,8 @@ def run_form_pcie(frames_mm, pose0, params, fmt, W, H, K, warmup, steps):
     fb = W * H * 2
     host = ctypes.c_void_p()
     N.check(N.lib.azu_host_alloc(ctypes.byref(host), fb * n))
-    ctypes.memmove(host.value, np.ascontiguousarray(frames_mm[:n]).ctypes.data, fb * n)
+    frames_c = np.ascontiguousarray(frames_mm[:n])  # referenced until the copy has run
+    ctypes.memmove(host.value, frames_c.ctypes.data, fb * n)
     slot = N.DeviceBuffer(fb)
     loop = FrameLoop(params, fmt, W, H, K)
     loop.reset(pose0)

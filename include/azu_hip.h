@@ -231,6 +231,15 @@ int azu_mesh_destroy(azu_mesh* m);
 int azu_tracker_create(int W, int H, const float K[4], const azu_icp_params* p, azu_tracker** out);
 int azu_tracker_destroy(azu_tracker* t);
 int azu_tracker_set_params(azu_tracker* t, const azu_icp_params* p);
+/* ICPTracker::params (ICPTracker.h:128): the parameters as sanitised. */
+int azu_tracker_get_params(const azu_tracker* t, azu_icp_params* out);
+/* ICPTracker::sanitizeParams (src/tracking/ICPTracker.cpp:24-34), what the constructor
+ * and setParams apply: the angle threshold clamps to [0, 85] degrees and a non-finite
+ * one becomes 30; no other field changes. Host computation only. */
+int azu_icp_params_sanitize(const azu_icp_params* in, azu_icp_params* out);
+/* ICPResult{} (ICPTracker.h:27-47): identity pose, final_step +inf, everything else 0.
+ * Host computation only. */
+int azu_icp_result_default(azu_icp_result* out);
 /* ICPTracker::prepareLiveGPU (ICPTracker.h:108) with the CPU semantics of buildFrameData
  * + computeNormals + buildFramePyramid (src/sensor/FrameData.cpp:113-345): depth in
  * uint16 millimetres (x 0.001 m) or float metres, band-gated to [min_depth, max_depth]. */

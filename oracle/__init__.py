@@ -56,6 +56,8 @@ lib.orc_mc_extract.argtypes = [P, P, I, F, P, LL, P, P, P, P, P, P]
 
 
 def _d(a):
+    """Address of a numpy array's data. The caller keeps the array referenced until the
+    C call returns: the address of a temporary is freed (and may be reused) at once."""
     return a.ctypes.data
 
 
@@ -107,7 +109,8 @@ class Volume:
         H, W = d.shape
         Kc = np.asarray(K, np.float32)
         newly = ctypes.c_longlong()
-        n = self.fn("integrate")(self.h, _d(d), W, H, _d(Kc), _d(cm(T)), min_depth, max_depth, ctypes.byref(newly))
+        Tc = cm(T)
+        n = self.fn("integrate")(self.h, _d(d), W, H, _d(Kc), _d(Tc), min_depth, max_depth, ctypes.byref(newly))
         return n, newly.value
 
     def raycast(self, T, K, W, H):
@@ -115,12 +118,14 @@ class Volume:
         nn = np.empty((H, W, 3), self.dtype)
         ns = ctypes.c_longlong()
         Kc = np.asarray(K, np.float32)
-        self.fn("raycast")(self.h, _d(cm(T)), _d(Kc), W, H, _d(v), _d(nn), ctypes.byref(ns))
+        Tc = cm(T)
+        self.fn("raycast")(self.h, _d(Tc), _d(Kc), W, H, _d(v), _d(nn), ctypes.byref(ns))
         return v, nn
 
     def raycast_touched(self, T, K, W, H):
         Kc = np.asarray(K, np.float32)
-        return self.fn("raycast_touched")(self.h, _d(cm(T)), _d(Kc), W, H)
+        Tc = cm(T)
+        return self.fn("raycast_touched")(self.h, _d(Tc), _d(Kc), W, H)
 
     def run_sequence(self, frames_mm, K, pose0, iters=(10, 5, 4), dist=0.1, angle=30.0, poses_in=None,
                      velocity=False):
@@ -135,7 +140,8 @@ class Volume:
         if poses_in is not None:
             pin = np.ascontiguousarray(np.stack([cm(T) for T in poses_in]), np.float32)
         icp = np.zeros((F_, 4), np.float64)  # converged, final_step, inliers, packed steps per level
-        self.fn("run_sequence")(self.h, _d(fr), F_, W, H, _d(Kc), _d(cm(pose0)), _d(it), dist, angle,
+        p0 = cm(pose0)
+        self.fn("run_sequence")(self.h, _d(fr), F_, W, H, _d(Kc), _d(p0), _d(it), dist, angle,
                                 _d(pin) if pin is not None else None, _d(poses_out), _d(ms), _d(nupd),
                                 1 if velocity else 0, _d(icp))
         self.last_icp = icp
@@ -189,8 +195,9 @@ def icp_system(lv, ln, mv, mn, K, T, ref, dist=0.1, angle=30.0, dtype=np.float32
     res = np.empty(1, dtype)
     cnt = np.empty(6, np.int32)
     Kc = np.asarray(K, np.float32)
-    ok = getattr(lib, "orc_icp_system_" + sfx)(_d(lv), _d(ln), W, H, _d(mv), _d(mn), MW, MH, _d(Kc), _d(cm(T)),
-                                                 _d(cm(ref)), dist, angle, _d(A), _d(b), _d(res), _d(cnt))
+    Tc, Rc = cm(T), cm(ref)
+    ok = getattr(lib, "orc_icp_system_" + sfx)(_d(lv), _d(ln), W, H, _d(mv), _d(mn), MW, MH, _d(Kc), _d(Tc),
+                                                 _d(Rc), dist, angle, _d(A), _d(b), _d(res), _d(cnt))
     return bool(ok), A.reshape(6, 6), b, float(res[0]), dict(zip(
         ["valid_live", "valid_model", "projected", "dist_filtered", "angle_filtered", "inliers"], cnt.tolist()))
 
@@ -209,8 +216,9 @@ def icp_track(levels_v, levels_n, mv, mn, K, est, ref, iters=(10, 5, 4), dist=0.
     sc = np.empty(2, dtype)
     ints = np.empty(9, np.int32)
     info = np.empty(36, dtype)
+    Ec, Rc = cm(est), cm(ref)  # kept alive across the call: _d() takes the address only
     getattr(lib, "orc_icp_track_" + sfx)(_d(lv[0]), _d(ln[0]), _d(lv[1]), _d(ln[1]), _d(lv[2]), _d(ln[2]), W, H,
-                                          _d(mv), _d(mn), MW, MH, _d(Kc), _d(cm(est)), _d(cm(ref)), _d(it), dist,
+                                          _d(mv), _d(mn), MW, MH, _d(Kc), _d(Ec), _d(Rc), _d(it), dist,
                                           angle, _d(pose), _d(sc), _d(ints), _d(info))
     keys = ["inliers", "converged", "tracking_ok", "valid_live_points", "valid_model_points", "projected_points",
             "dist_filtered", "angle_filtered", "steps"]

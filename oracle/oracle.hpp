@@ -133,7 +133,7 @@ struct Intr {
     float fx, fy, cx, cy;
 };
 
-// scaleIntrinsics (include/sensor/CameraIntrinsics.h:85-92)
+// scaleIntrinsics (include/sensor/CameraIntrinsics.h:33-40)
 inline Intr scaleIntrinsics(Intr k, int ref_w, int ref_h, int w, int h) {
     if (w == ref_w && h == ref_h) return k;
     const float sx = static_cast<float>(w) / static_cast<float>(ref_w);
@@ -193,7 +193,7 @@ void vertices(const T* depth, int W, int H, Intr K, V3<T>* v) {
     }
 }
 
-// depthJumpThreshold (include/tracking/ICPShared.h:257-262)
+// depthJumpThreshold (include/tracking/ICPShared.h:53-55)
 template <class T> inline T depthJump(T d) { return smax(static_cast<T>(0.03f), d * static_cast<T>(0.05f)); }
 
 // computeNormals (src/sensor/FrameData.cpp:196-251). Border pixels stay zero.
@@ -690,7 +690,7 @@ bool buildLinearSystem(const Level<T>& live, const V3<T>* mv, const V3<T>* mn, i
 
 // Cyclic Jacobi eigenvalues of a symmetric n x n matrix, in double (textbook method,
 // Golub & Van Loan 8.5). Eigen's SelfAdjointEigenSolver is the reference's routine
-// (ICPShared.h:293-307); only the extreme eigenvalues are used.
+// (ICPShared.h:86-100); only the extreme eigenvalues are used.
 inline void jacobiEigenvalues(const double* Ain, int n, double* eig) {
     double a[36];
     for (int i = 0; i < n * n; ++i) a[i] = Ain[i];
@@ -725,7 +725,7 @@ inline void jacobiEigenvalues(const double* Ain, int n, double* eig) {
     std::sort(eig, eig + n);
 }
 
-// dampingForHessian (include/tracking/ICPShared.h:293-307)
+// dampingForHessian (include/tracking/ICPShared.h:86-100)
 template <class T> inline T dampingForHessian(const T A[36], int level) {
     double Ad[36], eig[6];
     for (int i = 0; i < 36; ++i) Ad[i] = static_cast<double>(A[i]);
@@ -785,7 +785,7 @@ template <class T> inline bool ldltSolve(const T Ain[36], const T bin[6], T x[6]
     return true;
 }
 
-// Nearest rotation to M (projectToSO3, include/tracking/ICPShared.h:309-317): U V^T
+// Nearest rotation to M (projectToSO3, include/tracking/ICPShared.h:102-110): U V^T
 // from a one-sided Jacobi SVD in double (Hestenes), with the column of U that belongs
 // to the smallest singular value negated when det(U V^T) < 0.
 inline void projectToSO3d(const double M[9], double R[9]) {

@@ -67,8 +67,9 @@ class LossProblem:
         gbuf = None
         if backward:
             gbuf = N.DeviceBuffer(self.res ** 3 * 4)
-            N.check(N.lib.azu_memcpy_h2d(gbuf.ptr, np.zeros(self.res ** 3, np.float32).ctypes.data,
-                                         self.res ** 3 * 4, None))
+            zeros = np.zeros(self.res ** 3, np.float32)  # referenced until the copy has run
+            N.check(N.lib.azu_memcpy_h2d(gbuf.ptr, zeros.ctypes.data, zeros.nbytes, None))
+            N.synchronize()
         N.check(N.lib.azu_raycast_depth_loss(self.vol.h, N.fptr(pose_cm(T)), N.fptr(self.K), self.W, self.H,
                                              self.obs.ptr, self.step, 1 if backward else 0,
                                              gbuf.ptr if gbuf else None, None, out, None))

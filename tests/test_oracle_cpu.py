@@ -49,7 +49,7 @@ def test_icp_system_matches_hand_sums():
     Ar = np.zeros((6, 6))
     br = np.zeros(6)
     rr = 0.0
-    k = float(np.float32(0.02))  # kHuberK is the float constant 0.02f (ICPShared.h:241)
+    k = float(np.float32(0.02))  # kHuberK is the float constant 0.02f (ICPShared.h:34)
     for (py, px) in np.argwhere(mn[..., 2] == 1):
         v = lv[0][py, px].astype(np.float64)
         e = float(v[2] - mv[py, px, 2])
@@ -168,7 +168,7 @@ def test_project_so3_vs_svd():
 
 def test_damping_policy():
     """icp_numeric_policy_contract: 0.01 at level 0, 0.1 above, 1.0 when singular or
-    ill-conditioned (ICPShared.h:293-307)."""
+    ill-conditioned (ICPShared.h:86-100)."""
     A = np.ascontiguousarray(np.diag([1.0, 2, 3, 4, 5, 6]).astype(np.float32))
     assert oracle.lib.orc_damping_f32(A.ctypes.data, 0) == np.float32(0.01)
     assert oracle.lib.orc_damping_f32(A.ctypes.data, 1) == np.float32(0.1)
