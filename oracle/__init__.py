@@ -51,6 +51,8 @@ lib.orc_floor_to_int.argtypes = [F, P]
 lib.orc_scale_intrinsics.argtypes = [P, I, I, I, I, P]
 lib.orc_num_threads.restype = I
 lib.orc_set_icp_chunks.argtypes = [I]
+lib.orc_predict_pose_f32.argtypes = [P, P, P]
+lib.orc_predict_pose_f64.argtypes = [P, P, P]
 lib.orc_mc_extract.restype = LL
 lib.orc_mc_extract.argtypes = [P, P, I, F, P, LL, P, P, P, P, P, P]
 
@@ -157,6 +159,14 @@ class Volume:
             self.close()
         except Exception:
             pass
+
+
+def predict_pose(cur, last, dtype=np.float32):
+    """The pipeline's constant-velocity estimate cur * (last^-1 * cur), 4x4 row-major."""
+    c, l = cm(cur, dtype), cm(last, dtype)
+    out = np.empty(16, dtype)
+    getattr(lib, "orc_predict_pose_" + ("f32" if np.dtype(dtype) == np.float32 else "f64"))(_d(c), _d(l), _d(out))
+    return rm(out)
 
 
 def depth_mm(mm, min_d=0.3, max_d=5.0, dtype=np.float32):

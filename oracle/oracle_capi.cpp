@@ -257,6 +257,14 @@ ORC_API void orc_scale_intrinsics(const float* K, int rw, int rh, int w, int h, 
     out[0] = k.fx; out[1] = k.fy; out[2] = k.cx; out[3] = k.cy;
 }
 ORC_API void orc_set_icp_chunks(int n) { icpChunks() = n; }
+// The constant-velocity estimate (PipelineController.cpp:1162-1187) of two
+// column-major rigid poses: out = cur * (last^-1 * cur).
+ORC_API void orc_predict_pose_f32(const float* cur, const float* last, float* out) {
+    poseTo(predictPose(poseFrom<float>(cur), poseFrom<float>(last)), out);
+}
+ORC_API void orc_predict_pose_f64(const double* cur, const double* last, double* out) {
+    poseTo(predictPose(poseFrom<double>(cur), poseFrom<double>(last)), out);
+}
 // One downsample step (FrameData.cpp:262-334) on explicit level arrays.
 ORC_API void orc_downsample_f32(const float* d, const float* v, const float* n, int W, int H, float* d1, float* v1,
                                 float* n1) {
