@@ -199,6 +199,36 @@ Intr scaleIntrinsics(Intr k, int rw, int rh, int w, int h) {
 
 }  // namespace
 
+// Device scratch of azu_integrate_noalloc: a file-scope array every GPU holds its own
+// copy of until the process ends. Host code never passes the array's name to a kernel
+// (on the host it is only a stand-in for the symbol); it passes the device address that
+// hipGetSymbolAddress returns for the current device.
+constexpr size_t kNoAllocScratchBytes = (size_t)4 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's address of `scratch`, looked up once per device with that
+// device current.
+int scratchAddress(void** out) {
+    int dev = 0;
+    AZU_HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
+    if (dev < 0 || dev >= kMaxDevices) return fail(AZU_ERR_STATE, "device id out of range for the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return hipFail(e != hipSuccess ? e : hipErrorInvalidSymbol, "scratch address");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return AZU_OK;
+}
+}  // namespace
+
 struct azu_volume {
     azu_tsdf_params params;
     int format;
@@ -529,9 +559,12 @@ int azu_vol_integrated_frames(const azu_volume* v, int* frames) {
     return AZU_OK;
 }
 
+// bounds_scratch: null to use (and grow) the volume's own depth-bounds buffer, else a
+// caller-provided buffer of at least depthBoundsBytes(W, H).
 static int integrateImpl(azu_volume* v, const float* d_depth, int W, int H, const float* K, const PoseArg& pa,
-                         float min_d, float max_d, azu_stream s) {
+                         float min_d, float max_d, azu_stream s, void* bounds_scratch = nullptr) {
     if (!v || !d_depth) return fail(AZU_ERR_ARG, "null volume or depth");
+    if (!v->vox) return fail(AZU_ERR_STATE, "volume has no storage");
     if (W <= 0 || H <= 0) return fail(AZU_ERR_ARG, "image size must be positive");
     if (!finiteK(K)) return fail(AZU_ERR_ARG, "intrinsics must be finite with non-zero focal lengths");
     const bool exact = v->exact();
@@ -540,7 +573,7 @@ static int integrateImpl(azu_volume* v, const float* d_depth, int W, int H, cons
     ProfScope ps(exact ? (v->format == AZU_FMT_F32 ? "integrate_exact_f32" : "integrate_exact_f16")
                        : (v->format == AZU_FMT_F32 ? "integrate_fast_f32" : "integrate_fast_f16"),
                  (hipStream_t)s);
-    if (exact) {
+    if (exact && !bounds_scratch) {
         const size_t need = depthBoundsBytes(W, H);
         if (need > v->bounds_bytes) {
             // grows only when the frame size does; the old buffer may still be read by
@@ -554,7 +587,8 @@ static int integrateImpl(azu_volume* v, const float* d_depth, int W, int H, cons
         }
     }
     AZU_HIP_TRY(launchIntegrate(v->format, exact, v->vox, toVol(v->params), Intr{K[0], K[1], K[2], K[3]}, W, H,
-                                d_depth, min_d, max_d, pa, v->dtr, v->recip, exact ? v->bounds : nullptr, v->counters,
+                                d_depth, min_d, max_d, pa, v->dtr, v->recip,
+                                exact ? (bounds_scratch ? bounds_scratch : v->bounds) : nullptr, v->counters,
                                 v->bricks, (hipStream_t)s),
                 "integrate launch");
     v->frames += 1;
@@ -571,6 +605,24 @@ int azu_integrate_dpose(azu_volume* v, const float* d_depth_m, int W, int H, con
                         float min_depth, float max_depth, azu_stream s) {
     if (!d_pose) return fail(AZU_ERR_ARG, "null device pose");
     return integrateImpl(v, d_depth_m, W, H, K, poseByPtr(d_pose), min_depth, max_depth, s);
+}
+
+int azu_integrate_noalloc(azu_volume* v, const float* d_depth_m, int W, int H, const float K[4], const float pose[16],
+                          float min_depth, float max_depth, azu_stream s) {
+    if (!pose) return fail(AZU_ERR_ARG, "null pose");
+    if (!v || !d_depth_m) return fail(AZU_ERR_ARG, "null volume or depth");
+    if (W <= 0 || H <= 0) return fail(AZU_ERR_ARG, "image size must be positive");
+    if ((long long)W * H > (1LL << 26)) return fail(AZU_ERR_ARG, "frame too large");
+    const size_t need = depthBoundsBytes(W, H);
+    if (need > kNoAllocScratchBytes)
+        return fail(AZU_ERR_ARG, "azu_integrate_noalloc: the depth bounds of a " + std::to_string(W) + "x" +
+                                     std::to_string(H) + " frame need " + std::to_string(need) +
+                                     " bytes, more than the " + std::to_string(kNoAllocScratchBytes) +
+                                     "-byte scratch");
+    void* sp = nullptr;
+    const int rc = scratchAddress(&sp);
+    if (rc != AZU_OK) return rc;
+    return integrateImpl(v, d_depth_m, W, H, K, poseByValue(pose), min_depth, max_depth, s, sp);
 }
 
 static int raycastImpl(azu_volume* v, const PoseArg& pa, const float* K, int W, int H, float* dv, float* dn,

@@ -121,6 +121,8 @@ _SIGS = {
     "azu_vol_integrated_frames": (ctypes.c_int, [c_void_p, ctypes.POINTER(ctypes.c_int)]),
     "azu_integrate": (ctypes.c_int, [c_void_p, c_void_p, ctypes.c_int, ctypes.c_int, c_float_p, c_float_p,
                                      ctypes.c_float, ctypes.c_float, c_void_p]),
+    "azu_integrate_noalloc": (ctypes.c_int, [c_void_p, c_void_p, ctypes.c_int, ctypes.c_int, c_float_p, c_float_p,
+                                             ctypes.c_float, ctypes.c_float, c_void_p]),
     "azu_integrate_dpose": (ctypes.c_int, [c_void_p, c_void_p, ctypes.c_int, ctypes.c_int, c_float_p, c_void_p,
                                            ctypes.c_float, ctypes.c_float, c_void_p]),
     "azu_raycast": (ctypes.c_int, [c_void_p, c_float_p, c_float_p, ctypes.c_int, ctypes.c_int, c_void_p, c_void_p,

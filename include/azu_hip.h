@@ -182,6 +182,18 @@ int azu_vol_point_cloud(azu_volume* v, float* d_points, unsigned long long capac
  * 169-176): d_depth_m is W*H float metres on the device. */
 int azu_integrate(azu_volume* v, const float* d_depth_m, int W, int H, const float K[4], const float pose[16],
                   float min_depth, float max_depth, azu_stream s);
+/* No counterpart in the reference: azu_integrate for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Same arguments and the same result bit for bit; it calls no hipMalloc /
+ * hipFree: the per-frame depth-bounds scratch of the exact kernels is a library-wide
+ * 4 MiB device array (one per GPU, held until the process ends) instead of the volume's
+ * own buffer. Limit: frames whose bounds need more than 4 MiB (about 1.57 Mpx: 1440 x
+ * 1080 fits, 1600 x 1200 does not) are rejected with AZU_ERR_ARG. Every call on one GPU shares
+ * that array: calls that may overlap (other streams, other threads) race and must be
+ * serialised by the caller. The scratch is fully rewritten before it is read, so
+ * nothing needs clearing between calls. */
+int azu_integrate_noalloc(azu_volume* v, const float* d_depth_m, int W, int H, const float K[4], const float pose[16],
+                          float min_depth, float max_depth, azu_stream s);
 /* Same, pose read from device memory (16 floats) when the kernel runs. */
 int azu_integrate_dpose(azu_volume* v, const float* d_depth_m, int W, int H, const float K[4], const float* d_pose,
                         float min_depth, float max_depth, azu_stream s);
