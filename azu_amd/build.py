@@ -4,6 +4,8 @@
   (hipcc, one object per source, linked as a shared library).
 * ``azu_amd/libazu_scene.so``: the synthetic depth generator (host-only, g++).
 * ``oracle/liboracle.so``: the CPU oracle used by the tests (make in oracle/).
+* ``tests/cpp/bin/host_classes``: the C++ host classes driven through a frame loop and the
+  reference's volume / tracker contracts (g++, linked against libazu_hip.so).
 
 Every HIP translation unit is compiled with -ffp-contract=off and IEEE-exact fp32
 division / square root (the FP32 parity kernels rely on both) and without packed-FP32
@@ -65,8 +67,21 @@ def build_oracle():
     return os.path.join(ROOT, "oracle", "liboracle.so")
 
 
+def build_test_programs():
+    """tests/cpp/bin/host_classes: a C++ program driving the host classes of include/
+    (run by tests/test_gpu_host_classes.py)."""
+    out_dir = os.path.join(ROOT, "tests", "cpp", "bin")
+    os.makedirs(out_dir, exist_ok=True)
+    exe = os.path.join(out_dir, "host_classes")
+    _run(["g++", "-std=c++17", "-O2", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+          os.path.join(ROOT, "tests", "cpp", "host_classes.cpp"), "-o", exe + ".tmp", "-L", HERE, "-lazu_hip",
+          "-Wl,-rpath,$ORIGIN/../../../azu_amd"])
+    os.replace(exe + ".tmp", exe)
+    return exe
+
+
 def build_all(verbose=False):
-    return [build_hip(verbose), build_scene(), build_oracle()]
+    return [build_hip(verbose), build_scene(), build_oracle(), build_test_programs()]
 
 
 if __name__ == "__main__":

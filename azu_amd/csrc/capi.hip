@@ -63,14 +63,17 @@ void icpStateRead(const void* hostCopy, float* pose, float* information, float* 
 hipError_t launchDownsample(int W, int H, float* const d[3], float* const v[3], float* const n[3], hipStream_t st);
 hipError_t launchVolConvert(int format, bool to_float, void* vox, float* tsdf, float* weight, size_t n,
                             hipStream_t st);
+hipError_t launchCountObserved(int format, const void* vox, size_t n, unsigned long long* out, hipStream_t st);
 }  // namespace azu
 
 using namespace azu;
 
 namespace {
 
-// Integration counters: 256 slot pairs of u64, summed when read (integrate.hip).
+// Integration counters: 256 slot pairs of u64, summed when read (integrate.hip); the
+// allocation holds one more u64 after them for azu_vol_usage.
 constexpr size_t kCounterBytes = 256 * 2 * sizeof(unsigned long long);
+constexpr size_t kCounterAlloc = kCounterBytes + 64;
 
 thread_local std::string g_err;
 
@@ -249,6 +252,7 @@ struct azu_volume {
     // the map is rebuilt from the voxels before every raycast.
     unsigned char* bricks = nullptr;
     bool external = false;
+    bool bricks_stale = false;  // voxels written by azu_vol_write_voxels since the last map rebuild
     bool skip = true;  // empty-space skipping in the raycast
     unsigned* dsafe = nullptr;  // skip bounds, two words per 2 x 2 pixel cell, then one flag word
     size_t dsafe_cells = 0;
@@ -383,7 +387,7 @@ int azu_vol_create(const azu_tsdf_params* p, int format, azu_volume** out) {
         delete v;
         return rc;
     }
-    if (hipMalloc(&v->counters, kCounterBytes) != hipSuccess) {
+    if (hipMalloc(&v->counters, kCounterAlloc) != hipSuccess) {
         (void)hipFree(v->vox);
         (void)hipFree(v->bricks);
         delete v;
@@ -420,6 +424,7 @@ int azu_vol_destroy(azu_volume* v) {
 
 int azu_vol_reset(azu_volume* v, azu_stream s) {
     if (!v) return fail(AZU_ERR_ARG, "null volume");
+    if (!v->vox) return fail(AZU_ERR_STATE, "volume has no storage");
     const size_t n = (size_t)v->params.resolution * v->params.resolution * v->params.resolution;
     {
         ProfScope ps("fill_empty", (hipStream_t)s);
@@ -429,6 +434,7 @@ int azu_vol_reset(azu_volume* v, azu_stream s) {
     }
     AZU_HIP_TRY(hipMemsetAsync(v->counters, 0, kCounterBytes, (hipStream_t)s), "counter reset");
     v->frames = 0;
+    v->bricks_stale = false;
     return AZU_OK;
 }
 
@@ -438,16 +444,21 @@ int azu_vol_set_params(azu_volume* v, const azu_tsdf_params* p) {
     if (!validParams(p, &why)) return fail(AZU_ERR_ARG, why);
     if (sameParams(*p, v->params)) return AZU_OK;
     AZU_HIP_TRY(hipDeviceSynchronize(), "synchronise before setParams");
-    const bool realloc_ = p->resolution != v->params.resolution;
-    v->params = *p;
-    if (realloc_) {
+    if (p->resolution != v->params.resolution) {
+        // The new storage is allocated before the old one is released: if it fails, the
+        // volume keeps its storage, parameters and contents, and stays usable.
+        azu_volume fresh;
+        fresh.params = *p;
+        fresh.format = v->format;
+        const int rc = volAlloc(&fresh);
+        if (rc != AZU_OK) return rc;
         (void)hipFree(v->vox);
         (void)hipFree(v->bricks);
-        v->vox = nullptr;
-        v->bricks = nullptr;
-        const int rc = volAlloc(v);
-        if (rc != AZU_OK) return rc;
+        v->vox = fresh.vox;
+        v->bricks = fresh.bricks;
+        v->bytes = fresh.bytes;
     }
+    v->params = *p;
     const int rc = azu_vol_reset(v, nullptr);
     if (rc != AZU_OK) return rc;
     AZU_HIP_TRY(hipStreamSynchronize(nullptr), "volume reset");
@@ -489,6 +500,10 @@ int azu_vol_set_skip(azu_volume* v, int on) {
 
 static int volTransfer(azu_volume* v, float* tsdf, float* weight, bool download) {
     if (!v || !tsdf || !weight) return fail(AZU_ERR_ARG, "null argument");
+    if (!v->vox) return fail(AZU_ERR_STATE, "volume has no storage");
+    // Work on this volume may be queued on any stream (non-blocking ones included), which
+    // the null stream used below does not wait for.
+    AZU_HIP_TRY(hipDeviceSynchronize(), "synchronise before volume transfer");
     const size_t n = (size_t)v->params.resolution * v->params.resolution * v->params.resolution;
     const size_t chunk = std::min<size_t>(n, (size_t)1 << 24);
     float *dt = nullptr, *dw = nullptr;
@@ -518,6 +533,7 @@ static int volTransfer(azu_volume* v, float* tsdf, float* weight, bool download)
         hipError_t e = launchBrickMap(v->format, v->vox, v->params.resolution, v->bricks, true, nullptr);
         if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
         if (e != hipSuccess) rc = hipFail(e, "brick map rebuild");
+        else v->bricks_stale = false;
     }
     (void)hipFree(dt);
     (void)hipFree(dw);
@@ -527,6 +543,90 @@ static int volTransfer(azu_volume* v, float* tsdf, float* weight, bool download)
 int azu_vol_download(azu_volume* v, float* tsdf, float* weight) { return volTransfer(v, tsdf, weight, true); }
 int azu_vol_upload(azu_volume* v, const float* tsdf, const float* weight) {
     return volTransfer(v, const_cast<float*>(tsdf), const_cast<float*>(weight), false);
+}
+
+namespace {
+// Host form of the storage words (the device's StoreF32 / StoreF16 layout).
+float wordTsdf(int format, const unsigned char* w) {
+    if (format == AZU_FMT_F32) {
+        float t;
+        std::memcpy(&t, w, 4);
+        return t;
+    }
+    uint16_t h;
+    std::memcpy(&h, w, 2);
+    return __half2float(__ushort_as_half(h));
+}
+float wordWeight(int format, const unsigned char* w) {
+    if (format == AZU_FMT_F32) {
+        float x;
+        std::memcpy(&x, w + 4, 4);
+        return x;
+    }
+    uint16_t h;
+    std::memcpy(&h, w + 2, 2);
+    return (float)h;
+}
+}  // namespace
+
+int azu_vol_read_voxels(azu_volume* v, size_t first, size_t count, float* tsdf, float* weight) {
+    if (!v || (count && (!tsdf || !weight))) return fail(AZU_ERR_ARG, "null argument");
+    if (!v->vox) return fail(AZU_ERR_STATE, "volume has no storage");
+    const size_t n = (size_t)v->params.resolution * v->params.resolution * v->params.resolution;
+    if (first > n || count > n - first) return fail(AZU_ERR_ARG, "voxel range outside the volume");
+    if (!count) return AZU_OK;
+    const size_t wb = v->format == AZU_FMT_F32 ? 8 : 4;
+    std::vector<unsigned char> h(count * wb);
+    AZU_HIP_TRY(hipDeviceSynchronize(), "synchronise before voxel read");
+    AZU_HIP_TRY(hipMemcpy(h.data(), static_cast<char*>(v->vox) + first * wb, count * wb, hipMemcpyDeviceToHost),
+                "voxel read");
+    for (size_t i = 0; i < count; ++i) {
+        tsdf[i] = wordTsdf(v->format, &h[i * wb]);
+        weight[i] = wordWeight(v->format, &h[i * wb]);
+    }
+    return AZU_OK;
+}
+
+int azu_vol_write_voxels(azu_volume* v, size_t first, size_t count, const float* tsdf, const float* weight) {
+    if (!v || (count && (!tsdf || !weight))) return fail(AZU_ERR_ARG, "null argument");
+    if (!v->vox) return fail(AZU_ERR_STATE, "volume has no storage");
+    const size_t n = (size_t)v->params.resolution * v->params.resolution * v->params.resolution;
+    if (first > n || count > n - first) return fail(AZU_ERR_ARG, "voxel range outside the volume");
+    if (!count) return AZU_OK;
+    const size_t wb = v->format == AZU_FMT_F32 ? 8 : 4;
+    std::vector<unsigned char> h(count * wb);
+    for (size_t i = 0; i < count; ++i) {
+        unsigned char* w = &h[i * wb];
+        if (v->format == AZU_FMT_F32) {
+            std::memcpy(w, &tsdf[i], 4);
+            std::memcpy(w + 4, &weight[i], 4);
+        } else {
+            const float wt = weight[i];
+            if (!(wt >= 0.0f && wt <= 65535.0f) || wt != std::floor(wt))
+                return fail(AZU_ERR_ARG, "an F16_U16 volume stores integer weights in [0, 65535]");
+            const uint16_t ht = __half_as_ushort(__float2half_rn(tsdf[i]));
+            const uint16_t hw = (uint16_t)wt;
+            std::memcpy(w, &ht, 2);
+            std::memcpy(w + 2, &hw, 2);
+        }
+    }
+    AZU_HIP_TRY(hipDeviceSynchronize(), "synchronise before voxel write");
+    AZU_HIP_TRY(hipMemcpy(static_cast<char*>(v->vox) + first * wb, h.data(), count * wb, hipMemcpyHostToDevice),
+                "voxel write");
+    v->bricks_stale = true;  // the next raycast rebuilds the brick map from the voxels
+    return AZU_OK;
+}
+
+int azu_vol_usage(azu_volume* v, unsigned long long* observed) {
+    if (!v || !observed) return fail(AZU_ERR_ARG, "null argument");
+    if (!v->vox) return fail(AZU_ERR_STATE, "volume has no storage");
+    const size_t n = (size_t)v->params.resolution * v->params.resolution * v->params.resolution;
+    unsigned long long* slot = v->counters + kCounterBytes / sizeof(unsigned long long);
+    AZU_HIP_TRY(hipDeviceSynchronize(), "synchronise before usage count");
+    AZU_HIP_TRY(hipMemset(slot, 0, sizeof(unsigned long long)), "usage count clear");
+    AZU_HIP_TRY(launchCountObserved(v->format, v->vox, n, slot, nullptr), "usage count launch");
+    AZU_HIP_TRY(hipMemcpy(observed, slot, sizeof(unsigned long long), hipMemcpyDeviceToHost), "usage count read");
+    return AZU_OK;
 }
 
 int azu_vol_storage(azu_volume* v, void** d_ptr, size_t* bytes) {
@@ -628,6 +728,7 @@ int azu_integrate_noalloc(azu_volume* v, const float* d_depth_m, int W, int H, c
 static int raycastImpl(azu_volume* v, const PoseArg& pa, const float* K, int W, int H, float* dv, float* dn,
                        float* dt, azu_stream s) {
     if (!v || !dv || !dn) return fail(AZU_ERR_ARG, "null volume or output");
+    if (!v->vox) return fail(AZU_ERR_STATE, "volume has no storage");
     if (W <= 0 || H <= 0) return fail(AZU_ERR_ARG, "image size must be positive");
     if (!finiteK(K)) return fail(AZU_ERR_ARG, "intrinsics must be finite with non-zero focal lengths");
     const bool exact = v->exact();
@@ -646,9 +747,11 @@ static int raycastImpl(azu_volume* v, const PoseArg& pa, const float* K, int W, 
         AZU_HIP_TRY(launchSkipInit(v->dsafe, cells, (hipStream_t)s), "skip scratch init");
         v->dsafe_cells = cells;
     }
-    if (v->skip && v->external)
+    if (v->skip && (v->external || v->bricks_stale)) {
         AZU_HIP_TRY(launchBrickMap(v->format, v->vox, v->params.resolution, v->bricks, true, (hipStream_t)s),
                     "brick map rebuild");
+        v->bricks_stale = false;
+    }
     AZU_HIP_TRY(launchRaycast(v->format, exact, v->vox, toVol(v->params), v->dvs, v->skip ? v->bricks : nullptr,
                               v->dsafe,
                               reinterpret_cast<int*>(v->dsafe + 2 * v->dsafe_cells), pa, Intr{K[0], K[1], K[2], K[3]}, W,
@@ -673,6 +776,7 @@ int azu_raycast_depth_loss(azu_volume* v, const float pose[16], const float K[4]
                            const float* d_depth_obs, float step_voxels, int backward, float* d_grad_tsdf,
                            float* d_depth_out, double out[8], azu_stream s) {
     if (!v || !pose || !out) return fail(AZU_ERR_ARG, "null volume, pose or output");
+    if (!v->vox) return fail(AZU_ERR_STATE, "volume has no storage");
     if (W <= 0 || H <= 0) return fail(AZU_ERR_ARG, "image size must be positive");
     if (!finiteK(K)) return fail(AZU_ERR_ARG, "intrinsics must be finite with non-zero focal lengths");
     if (!(step_voxels >= 0.05f && step_voxels <= 8.0f)) return fail(AZU_ERR_ARG, "step_voxels must be in [0.05, 8]");
@@ -708,8 +812,11 @@ int azu_vol_point_cloud(azu_volume* v, float* d_points, unsigned long long capac
                         azu_stream s) {
     if (!v || !count) return fail(AZU_ERR_ARG, "null volume or count");
     if (capacity > 0 && !d_points) return fail(AZU_ERR_ARG, "null point buffer with a non-zero capacity");
+    if (!v->vox) return fail(AZU_ERR_STATE, "volume has no storage");
     if ((size_t)v->params.resolution * v->params.resolution * v->params.resolution >= ((size_t)1 << 31))
         return fail(AZU_ERR_ARG, "point cloud: resolution^3 must stay below 2^31");
+    // The null stream does not wait for work queued on non-blocking streams.
+    if (!s) AZU_HIP_TRY(hipDeviceSynchronize(), "synchronise before point cloud");
     ProfScope ps("point_cloud", (hipStream_t)s);
     AZU_HIP_TRY(launchPointCloud(v->format, v->vox, toVol(v->params), d_points, capacity, count, (hipStream_t)s),
                 "point cloud");
@@ -719,6 +826,9 @@ int azu_vol_point_cloud(azu_volume* v, float* d_points, unsigned long long capac
 int azu_mc_extract(azu_volume* v, unsigned long long max_triangles, azu_mesh** out, azu_stream s) {
     if (!v || !out) return fail(AZU_ERR_ARG, "null volume or output");
     *out = nullptr;
+    if (!v->vox) return fail(AZU_ERR_STATE, "volume has no storage");
+    // The null stream does not wait for work queued on non-blocking streams.
+    if (!s) AZU_HIP_TRY(hipDeviceSynchronize(), "synchronise before marching cubes");
     auto* m = new azu_mesh();
     hipError_t e;
     {
@@ -995,16 +1105,17 @@ int azu_track_grade(const azu_icp_result* r, const float prev_pose[16], const az
         pol.failures_before_lost = p->failures_before_lost;
     }
     T::ICPResult res;
-    std::memcpy(res.pose, r->pose, sizeof(res.pose));
-    std::memcpy(res.information, r->information, sizeof(res.information));
+    std::memcpy(res.pose.data(), r->pose, sizeof(r->pose));
+    std::memcpy(res.information.data(), r->information, sizeof(r->information));
     res.error = r->error;
     res.inliers = r->inliers;
     res.valid_live_points = r->valid_live_points;
     res.valid_model_points = r->valid_model_points;
     int dofs = 0;
     if (degeneracy_rel > 0.0f) {
-        const T::ObservableMotion om = T::keepObservableMotion(prev_pose, res.pose, res.information, degeneracy_rel);
-        std::memcpy(res.pose, om.pose, sizeof(res.pose));
+        const T::ObservableMotion om =
+            T::keepObservableMotion(prev_pose, res.pose.data(), res.information.data(), degeneracy_rel);
+        res.pose = om.pose;
         dofs = om.degenerate_dofs;
     }
     const T::TrackQuality q = T::classifyTracking(res, prev_pose, pol);
@@ -1012,7 +1123,7 @@ int azu_track_grade(const azu_icp_result* r, const float prev_pose[16], const az
                                               : (q == T::TrackQuality::Poor ? AZU_TRACK_POOR : AZU_TRACK_FAILED);
     out->degenerate_dofs = dofs;
     out->integrate = q == T::TrackQuality::Good ? 1 : 0;
-    std::memcpy(out->pose, q == T::TrackQuality::Failed ? prev_pose : res.pose, sizeof(out->pose));
+    std::memcpy(out->pose, q == T::TrackQuality::Failed ? prev_pose : res.pose.data(), sizeof(out->pose));
     return AZU_OK;
 }
 

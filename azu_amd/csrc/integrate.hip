@@ -791,6 +791,30 @@ __global__ void volConvert(typename S::Word* __restrict__ vox, float* __restrict
     }
 }
 
+// Voxels with weight > 0 (TSDFVolume::usageFraction's count), one wave sum per 64
+// voxels and one atomic per block into *out (cleared by the caller).
+template <class S>
+__global__ __launch_bounds__(256) void countObserved(const typename S::Word* __restrict__ vox, size_t n,
+                                                     unsigned long long* __restrict__ out) {
+    unsigned long long c = 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        c += S::weight(vox[i]) > 0.0f ? 1u : 0u;
+    __shared__ unsigned long long part[4];
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(out, part[0] + part[1] + part[2] + part[3]);
+}
+
+hipError_t launchCountObserved(int format, const void* vox, size_t n, unsigned long long* out, hipStream_t st) {
+    const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+    if (format == kFmtF32)
+        hipLaunchKernelGGL(countObserved<StoreF32>, dim3(blocks), dim3(256), 0, st, (const float2*)vox, n, out);
+    else
+        hipLaunchKernelGGL(countObserved<StoreF16>, dim3(blocks), dim3(256), 0, st, (const uint32_t*)vox, n, out);
+    return hipGetLastError();
+}
+
 hipError_t launchVolConvert(int format, bool to_float, void* vox, float* tsdf, float* weight, size_t n,
                             hipStream_t st) {
     const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 65536);

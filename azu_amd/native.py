@@ -115,6 +115,9 @@ _SIGS = {
     "azu_vol_division_check": (ctypes.c_int, [c_void_p] + [ctypes.POINTER(ctypes.c_int)] * 3),
     "azu_vol_download": (ctypes.c_int, [c_void_p, c_void_p, c_void_p]),
     "azu_vol_upload": (ctypes.c_int, [c_void_p, c_void_p, c_void_p]),
+    "azu_vol_read_voxels": (ctypes.c_int, [c_void_p, ctypes.c_size_t, ctypes.c_size_t, c_void_p, c_void_p]),
+    "azu_vol_write_voxels": (ctypes.c_int, [c_void_p, ctypes.c_size_t, ctypes.c_size_t, c_void_p, c_void_p]),
+    "azu_vol_usage": (ctypes.c_int, [c_void_p, ctypes.POINTER(ctypes.c_ulonglong)]),
     "azu_vol_storage": (ctypes.c_int, [c_void_p, ctypes.POINTER(c_void_p), ctypes.POINTER(ctypes.c_size_t)]),
     "azu_vol_counters": (ctypes.c_int, [c_void_p, ctypes.POINTER(ctypes.c_ulonglong),
                                         ctypes.POINTER(ctypes.c_ulonglong), ctypes.c_int]),

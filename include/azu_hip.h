@@ -139,7 +139,9 @@ int azu_vol_destroy(azu_volume* v);
 /* TSDFVolume::reset (TSDFVolume.cpp:112-122). */
 int azu_vol_reset(azu_volume* v, azu_stream s);
 /* TSDFVolume::setParams (TSDFVolume.cpp:124-146): any field difference clears the
- * volume; a resolution change reallocates. Synchronises. */
+ * volume; a resolution change reallocates (the new storage first: if that fails, the
+ * call returns AZU_ERR_ALLOC and the volume keeps its parameters and contents).
+ * Synchronises. */
 int azu_vol_set_params(azu_volume* v, const azu_tsdf_params* p);
 int azu_vol_get_params(const azu_volume* v, azu_tsdf_params* p, int* format);
 int azu_vol_set_mode(azu_volume* v, int mode);
@@ -156,10 +158,21 @@ int azu_vol_set_skip(azu_volume* v, int on);
  * |x| < 2^32; every finite float); results are the same either way. Any pointer may be
  * NULL. No counterpart in the reference. */
 int azu_vol_division_check(const azu_volume* v, int* voxel_size_fast, int* truncation_fast, int* reciprocal_fast);
-/* Whole-volume copies to / from host float arrays of res^3 (voxelData / voxelAt).
- * Synchronise. */
+/* Whole-volume copies to / from host float arrays of res^3 (voxelData, syncFromGPU /
+ * syncToGPU, TSDFVolume.h:134, 161-162). They wait for all work queued on the device
+ * (any stream) first, then synchronise. */
 int azu_vol_download(azu_volume* v, float* tsdf, float* weight);
 int azu_vol_upload(azu_volume* v, const float* tsdf, const float* weight);
+/* voxelAt (TSDFVolume.h:117-118) without a whole-volume copy: count voxels from linear
+ * index `first` (z*res*res + y*res + x) read to / written from host floats. F16_U16
+ * volumes store the tsdf rounded to binary16 and need integer weights in [0, 65535].
+ * A write is seen by every later call (the next raycast first rebuilds its empty-space
+ * map from the voxels). Both wait for all work queued on the device, then synchronise. */
+int azu_vol_read_voxels(azu_volume* v, size_t first, size_t count, float* tsdf, float* weight);
+int azu_vol_write_voxels(azu_volume* v, size_t first, size_t count, const float* tsdf, const float* weight);
+/* TSDFVolume::usageFraction's count (TSDFVolume.cpp:563-569): voxels with weight > 0.
+ * Waits for the device, synchronises. */
+int azu_vol_usage(azu_volume* v, unsigned long long* observed);
 /* Raw device storage (format-dependent words, index as above). The raycast skips empty
  * space with a brick map that integrate / reset / upload keep; once the storage has been
  * handed out, writes through it are possible and every later raycast of this volume
@@ -169,7 +182,8 @@ int azu_vol_storage(azu_volume* v, void** d_ptr, size_t* bytes);
  * (reset != 0 clears them after reading). Synchronises. */
 int azu_vol_counters(azu_volume* v, unsigned long long* updated, unsigned long long* newly, int reset);
 int azu_vol_integrated_frames(const azu_volume* v, int* frames);
-/* TSDFVolume::extractGlobalPointCloud (TSDFVolume.cpp:571-632; its GPU form
+/* (With s NULL, this and azu_mc_extract first wait for all work queued on the device.)
+ * TSDFVolume::extractGlobalPointCloud (TSDFVolume.cpp:571-632; its GPU form
  * compactPointsKernel, TSDFVolume_cuda.cu:409-447): every voxel with weight > 1 and
  * |tsdf| < 0.2, as origin + (x, y, z) * voxel_size, xyz float triplets in linear voxel
  * order (x fastest). Writes min(count, capacity) points to d_points (device memory for

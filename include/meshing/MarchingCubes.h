@@ -16,14 +16,17 @@
 #include <string>
 
 #include "../azu_hip.h"
+#include "../azu_types.h"
 #include "../tsdf/TSDFVolume.h"
 
 namespace kfusion {
 namespace meshing {
 
+// MeshData (MeshData.h:13-40): positions / normals are Eigen::Vector3f (kfusion::Vec3f),
+// packed xyz floats.
 struct MeshData {
-    std::vector<std::array<float, 3>> positions;
-    std::vector<std::array<float, 3>> normals;
+    std::vector<Vec3f> positions;
+    std::vector<Vec3f> normals;
     std::vector<uint8_t> colors;
     std::vector<uint32_t> indices;
     bool truncated = false;
@@ -50,7 +53,7 @@ struct MeshData {
             if (reason) *reason = why;
             return false;
         };
-        auto finite3 = [](const std::array<float, 3>& v) {
+        auto finite3 = [](const Vec3f& v) {
             return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]);
         };
         const size_t nv = positions.size(), ni = indices.size();
@@ -84,10 +87,12 @@ public:
     void setMaxTriangles(size_t n) { max_triangles_ = n; }
     size_t maxTriangles() const { return max_triangles_; }
 
-    // extractGPU: the mesh on the device, downloaded into MeshData.
+    // extractGPU: the mesh on the device, downloaded into MeshData. Runs on the volume's
+    // stream (TSDFVolume::setStream), after the work queued there.
     std::shared_ptr<MeshData> extractGPU(const tsdf::TSDFVolume& volume) {
         azu_mesh* m = nullptr;
-        detail::check(azu_mc_extract(volume.handle(), max_triangles_, &m, nullptr), "MarchingCubes::extractGPU");
+        detail::check(azu_mc_extract(volume.handle(), max_triangles_, &m, volume.stream()),
+                      "MarchingCubes::extractGPU");
         auto out = std::make_shared<MeshData>();
         unsigned long long nv = 0, nt = 0;
         int tr = 0;

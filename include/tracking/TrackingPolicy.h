@@ -1,9 +1,9 @@
 // kfusion::tracking frame-level tracking decision for the MI355X backend: header-only host
 // C++ over the ICPResult that ICPTracker::trackGPU reads back once per frame, shaped like
 // the reference's include/tracking/TrackingPolicy.h:36-161 (iphands/azu): same names, same
-// defaults, same grading. Poses are column-major 4x4 float arrays (ICPResult::pose,
-// Eigen::Matrix4f::data()); the information matrix is ICPResult::information (6x6,
-// symmetric, [t; w] order).
+// defaults, same grading. Poses are Eigen::Matrix4f (kfusion::Mat4f) or column-major 4x4
+// float arrays; the information matrix is ICPResult::information (6x6, symmetric,
+// [t; w] order).
 //
 //   Good   : finite, enough inliers, tight fit on the model correspondences, motion inside
 //            the per-frame gate -> update the pose and integrate
@@ -190,7 +190,7 @@ inline float icpFitRatio(const ICPResult& r) {
 
 // The fit alone, without the motion gate (TrackingPolicy.h:69-78).
 inline TrackQuality classifyFit(const ICPResult& r, const TrackingPolicy& p = TrackingPolicy{}) {
-    if (!policy_detail::allFinite(r.pose, 16) || !std::isfinite(r.error) || r.inliers < p.min_inliers)
+    if (!policy_detail::allFinite(r.pose.data(), 16) || !std::isfinite(r.error) || r.inliers < p.min_inliers)
         return TrackQuality::Failed;
     if (r.inliers >= p.min_good_inliers && icpFitRatio(r) >= p.min_fit_ratio && icpRmsMeters(r) <= p.max_rms_m)
         return TrackQuality::Good;
@@ -213,7 +213,7 @@ inline TrackQuality classifyTracking(const ICPResult& r, const float* prev_pose,
                                      const TrackingPolicy& p = TrackingPolicy{}) {
     const TrackQuality fit = classifyFit(r, p);
     if (fit == TrackQuality::Failed) return fit;
-    if (!withinFrameMotion(r.pose, prev_pose, p)) return TrackQuality::Failed;
+    if (!withinFrameMotion(r.pose.data(), prev_pose, p)) return TrackQuality::Failed;
     return fit;
 }
 
@@ -231,7 +231,7 @@ inline float weakestDirectionRatio(const float* information) {
 
 // ObservableMotion (TrackingPolicy.h:121-124): the pose with unobserved motion removed.
 struct ObservableMotion {
-    float pose[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+    Mat4f pose = Mat4f::Identity();
     int degenerate_dofs = 0;
 };
 
@@ -244,7 +244,7 @@ struct ObservableMotion {
 inline ObservableMotion keepObservableMotion(const float* prev, const float* est, const float* information,
                                              float rel_threshold = 5e-3f) {
     ObservableMotion out;
-    for (int i = 0; i < 16; ++i) out.pose[i] = est[i];
+    for (int i = 0; i < 16; ++i) out.pose.data()[i] = est[i];
     if (!policy_detail::allFinite(information, 36)) return out;
     double a[36], ev[6], v[36];
     for (int i = 0; i < 36; ++i) a[i] = information[i];
@@ -290,7 +290,7 @@ inline ObservableMotion keepObservableMotion(const float* prev, const float* est
     double pv[16], res[16];
     for (int i = 0; i < 16; ++i) pv[i] = prev[i];
     policy_detail::mul4(pv, step, res);
-    for (int i = 0; i < 16; ++i) out.pose[i] = static_cast<float>(res[i]);
+    for (int i = 0; i < 16; ++i) out.pose.data()[i] = static_cast<float>(res[i]);
     return out;
 }
 

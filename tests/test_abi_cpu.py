@@ -91,8 +91,9 @@ int run(const float* d_depth, int W, int H, int frames) {
             tracking::ICPResult r = icp.trackGPU(d_depth, nullptr, W, H, model, pose, model.pose);
             const tracking::TrackQuality q = tracking::classifyTracking(r, pose);
             if (q != tracking::TrackQuality::Failed) {
-                const tracking::ObservableMotion m = tracking::keepObservableMotion(pose.m, r.pose, r.information);
-                for (int i = 0; i < 16; ++i) pose.m[i] = m.pose[i];
+                const tracking::ObservableMotion m = tracking::keepObservableMotion(pose.m, r.pose.data(),
+                                                                                    r.information.data());
+                for (int i = 0; i < 16; ++i) pose.m[i] = m.pose.data()[i];
             }
             integrate = q == tracking::TrackQuality::Good;
         }
@@ -100,7 +101,7 @@ int run(const float* d_depth, int W, int H, int frames) {
     }
     meshing::MarchingCubes mc;
     auto mesh = mc.extract(vol, [](float x) { std::printf("%f\n", x); });
-    std::vector<std::array<float, 3>> cloud;
+    std::vector<Vec3f> cloud;
     std::vector<uint8_t> colors;
     vol.extractGlobalPointCloud(cloud, colors);
     export_io::PLYExporter::writeBinary(*mesh, "mesh.ply");

@@ -26,7 +26,7 @@ extern "C" float tp_ratio(const float* info) { return weakestDirectionRatio(info
 extern "C" int tp_within(const float* pose, const float* prev) { return withinFrameMotion(pose, prev) ? 1 : 0; }
 extern "C" int tp_keep(const float* prev, const float* est, const float* info, float thr, float* out) {
     ObservableMotion m = keepObservableMotion(prev, est, info, thr);
-    for (int i = 0; i < 16; ++i) out[i] = m.pose[i];
+    for (int i = 0; i < 16; ++i) out[i] = m.pose.data()[i];
     return m.degenerate_dofs;
 }
 """

@@ -71,7 +71,7 @@ Mat4 rollMove(float tx, float ty, float tz, float roll) {
 
 ICPResult result(const Mat4& pose, int inliers, int valid_model, float error, int valid_live = 300000) {
     ICPResult r;
-    for (int i = 0; i < 16; ++i) r.pose[i] = pose.m[i];
+    for (int i = 0; i < 16; ++i) r.pose.data()[i] = pose.m[i];
     r.inliers = inliers;
     r.valid_model_points = valid_model;
     r.valid_live_points = valid_live;
@@ -165,24 +165,24 @@ int main() {
     // keepObservableMotion: full rank keeps the estimate as it is.
     const Mat4 est = prev * rollMove(0.03f, -0.02f, 0.01f, 0.05f);
     ObservableMotion om = keepObservableMotion(prev, est, diag6(50, 60, 70, 80, 90, 100));
-    EXPECT(om.degenerate_dofs == 0 && maxAbsDiff(om.pose, est.m) == 0.0f, "full rank: estimate unchanged");
+    EXPECT(om.degenerate_dofs == 0 && maxAbsDiff(om.pose.data(), est.m) == 0.0f, "full rank: estimate unchanged");
     // Roll about the optical axis unobserved (w_z): the kept motion is the translation.
     om = keepObservableMotion(prev, est, diag6(100, 100, 100, 100, 100, 0.1f));
     EXPECT(om.degenerate_dofs == 1, "one unobserved direction");
     const Mat4 want_t = prev * rollMove(0.03f, -0.02f, 0.01f, 0.0f);
-    EXPECT(maxAbsDiff(om.pose, want_t.m) < 2e-6f, "unobserved roll: translation kept, roll dropped");
+    EXPECT(maxAbsDiff(om.pose.data(), want_t.m) < 2e-6f, "unobserved roll: translation kept, roll dropped");
     // Sliding along x unobserved (t_x): the kept motion is the roll and the other translation.
     om = keepObservableMotion(prev, est, diag6(0.2f, 100, 100, 100, 100, 100));
     const Mat4 want_x = prev * rollMove(0.0f, -0.02f, 0.01f, 0.05f);
-    EXPECT(om.degenerate_dofs == 1 && maxAbsDiff(om.pose, want_x.m) < 2e-6f, "unobserved slide: x dropped");
+    EXPECT(om.degenerate_dofs == 1 && maxAbsDiff(om.pose.data(), want_x.m) < 2e-6f, "unobserved slide: x dropped");
     // Every direction below the threshold: the previous pose.
     om = keepObservableMotion(prev, est, diag6(1e-4f, 1e-4f, 1e-4f, 1e-4f, 1e-4f, 1));
     const Mat4 want_roll = prev * rollMove(0.0f, 0.0f, 0.0f, 0.05f);
-    EXPECT(om.degenerate_dofs == 5 && maxAbsDiff(om.pose, want_roll.m) < 2e-6f,
+    EXPECT(om.degenerate_dofs == 5 && maxAbsDiff(om.pose.data(), want_roll.m) < 2e-6f,
            "only the roll observed: roll kept, translation dropped");
     // A non-finite or empty system leaves the estimate as it is.
     om = keepObservableMotion(prev, est, Mat6{});
-    EXPECT(om.degenerate_dofs == 0 && maxAbsDiff(om.pose, est.m) == 0.0f, "empty system: estimate unchanged");
+    EXPECT(om.degenerate_dofs == 0 && maxAbsDiff(om.pose.data(), est.m) == 0.0f, "empty system: estimate unchanged");
 
     if (g_failures == 0) {
         std::printf("tracking_policy_contract: PASS (%d checks)\n", g_checks);
