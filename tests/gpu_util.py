@@ -50,13 +50,23 @@ def steps_per_level(packed):
     return tuple((int(packed) >> (8 * l)) & 255 for l in range(3))
 
 
-def icp_pose_parity(gpu_pose, gpu_steps, ref_pose, ref_steps):
-    """The ICP parity rule. When both runs take the same accepted steps per level the
-    poses agree within 1e-5 m / 1e-5 rad. The stop test |x| < 5e-5 is discrete, and the
-    reference's own result moves by one step when its input moves by one float ulp, so
-    when the step counts differ the poses must agree within one step: 1e-4 m / 1e-4 rad.
-    Returns (ok, dt, ang, same_steps)."""
+# The ICP parity rule. When both runs take the same accepted steps per level and both
+# converge (|x| < 5e-5 before the {10, 5, 4} budget runs out), the poses agree within the
+# north-star bar 1e-5 m / 1e-5 rad. The GPU's system differs from the oracle's in the
+# summation order only (~1e-7 relative), which can move the discrete stop test by one
+# iteration: then the poses differ by one step that the test accepts, below 5e-5; and on a
+# frame whose budget runs out unconverged the last iterate is still moving. For those
+# frames the bar is 5e-5 m / 5e-5 rad. (Measured on the 100-frame 512^3 sequence, open
+# loop: the oracle itself, summed with 1..64 ordered chunks as the reference's thread
+# count would, spreads by up to 6.4e-5 m / 1.6e-4 rad on such frames.)
+PARITY_TOL = 1e-5
+STOP_RULE_TOL = 5e-5
+
+
+def icp_pose_parity(gpu_pose, gpu_steps, ref_pose, ref_steps, converged=True):
+    """Returns (ok, dt, ang, same_steps): ok when the poses agree within PARITY_TOL, or
+    within STOP_RULE_TOL when the step counts differ or a run did not converge."""
     dt, ang = pose_diff(gpu_pose, ref_pose)
     same = steps_per_level(gpu_steps) == steps_per_level(ref_steps)
-    tol = 1e-5 if same else 1e-4
+    tol = PARITY_TOL if (same and converged) else STOP_RULE_TOL
     return (dt < tol and ang < tol), dt, ang, same

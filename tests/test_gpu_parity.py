@@ -91,7 +91,8 @@ def test_icp_single_frame_pose():
     rg = tr.track(mv, mn, T0, T0)
     say("oracle", ro["inliers"], ro["converged"], ro["final_step"], "gpu", rg["inliers"], rg["converged"],
         rg["final_step"], rg["iterations"])
-    ok, dt, ang, same = icp_pose_parity(rg["pose"], rg["iterations"], ro["pose"], ro["steps"])
+    ok, dt, ang, same = icp_pose_parity(rg["pose"], rg["iterations"], ro["pose"], ro["steps"],
+                                        rg["converged"] and bool(ro["converged"]))
     say("pose diff", dt, ang, "steps gpu", steps_per_level(rg["iterations"]), "oracle",
         steps_per_level(ro["steps"]), "true err", np.abs(rg["pose"][:3, 3] - T1[:3, 3]).max())
     assert ok, (dt, ang, same)

@@ -98,7 +98,8 @@ def test_c1_sequence_against_oracles():
         loop.frame(d_.ptr)
     Pg = loop.poses()
     loop.close()
-    ok, dt1, ang1, same = icp_pose_parity(r1["pose"], r1["iterations"], P["fp32"][1], int(icp["fp32"][1, 3]))
+    ok, dt1, ang1, same = icp_pose_parity(r1["pose"], r1["iterations"], P["fp32"][1], int(icp["fp32"][1, 3]),
+                                          r1["converged"] and icp["fp32"][1, 0] > 0)
     say("c1 frame 1: GPU vs FP32 oracle %.2e m %.2e rad, same steps %s" % (dt1, ang1, same))
     assert ok
     def err(A, B):
