@@ -43,6 +43,21 @@ class LossProblem:
         self.step = step_voxels
         self.res = vol.res
         self.tsdf, self.weight = vol.voxelData()
+        self._edited = set()
+
+    def set_voxels(self, idx, vals):
+        """The volume with voxels idx set to vals and every other voxel at its original
+        value (voxels edited by an earlier call and not in idx are restored), written
+        voxel by voxel (azu_vol_write_voxels) rather than by a whole-volume upload."""
+        flat_t, flat_w = self.tsdf.reshape(-1), self.weight.reshape(-1)
+        edits = dict(zip((int(i) for i in idx), (np.float32(v) for v in vals)))
+        for i in sorted(self._edited - set(edits)):
+            edits_i = np.array([flat_t[i]], np.float32), np.array([flat_w[i]], np.float32)
+            N.check(N.lib.azu_vol_write_voxels(self.vol.h, i, 1, edits_i[0].ctypes.data, edits_i[1].ctypes.data))
+        for i, v in edits.items():
+            t1, w1 = np.array([v], np.float32), np.array([flat_w[i]], np.float32)
+            N.check(N.lib.azu_vol_write_voxels(self.vol.h, i, 1, t1.ctypes.data, w1.ctypes.data))
+        self._edited = set(edits)
 
     def set_obs(self, d_obs):
         self.obs = N.device_array(N.f32(d_obs))
@@ -58,9 +73,7 @@ class LossProblem:
 
     def run(self, xi, backward=False, vals=None, idx=None):
         if vals is not None:
-            t = self.tsdf.reshape(-1).copy()
-            t[idx] = vals
-            self.vol.upload(t, self.weight)
+            self.set_voxels(np.atleast_1d(idx), np.atleast_1d(vals))
         T = se3_right(self.pose0, xi).astype(np.float32)
         out = (ctypes.c_double * 8)()
         grad = None
