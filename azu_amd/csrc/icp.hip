@@ -344,6 +344,61 @@ __device__ void projectSO3(const double* M, double* R) {
 }
 
 
+// Rodrigues' rotation of a rotation vector w with |w| <= 0.5 (the step cap), in FP64:
+// R = cos t I + a [w]x + b w w^T with t = |w|, a = sin t / t and b = (1 - cos t) / t^2
+// summed as their Taylor series in t^2 (seven terms: the first omitted one is below
+// 0.25^7 / 15! < 5e-17), cos t = 1 - t^2 b. No square root, division or trigonometric
+// call; the same matrix as the axis-angle form cI + s[k]x + (1 - c) k k^T to FP64 rounding.
+__device__ __forceinline__ void rodrigues64(double wx, double wy, double wz, double* R) {
+    const double t2 = wx * wx + wy * wy + wz * wz;
+    const double a = 1.0 + t2 * (-1.0 / 6.0 + t2 * (1.0 / 120.0 + t2 * (-1.0 / 5040.0 + t2 * (1.0 / 362880.0 +
+                     t2 * (-1.0 / 39916800.0 + t2 * (1.0 / 6227020800.0))))));
+    const double b = 0.5 + t2 * (-1.0 / 24.0 + t2 * (1.0 / 720.0 + t2 * (-1.0 / 40320.0 + t2 * (1.0 / 3628800.0 +
+                     t2 * (-1.0 / 479001600.0 + t2 * (1.0 / 87178291200.0))))));
+    const double c = 1.0 - t2 * b;
+    R[0] = c + b * wx * wx;
+    R[4] = c + b * wy * wy;
+    R[8] = c + b * wz * wz;
+    R[1] = b * wx * wy - a * wz;
+    R[3] = b * wx * wy + a * wz;
+    R[2] = b * wx * wz + a * wy;
+    R[6] = b * wx * wz - a * wy;
+    R[5] = b * wy * wz - a * wx;
+    R[7] = b * wy * wz + a * wx;
+}
+
+// The projection onto SO(3) of the products the update forms (a rotation times a
+// rotation, orthogonal to float rounding: |M^T M - I| <= 1e-6): one Newton-Schulz step
+// X = M (3I - M^T M) / 2 (Higham, Functions of Matrices, 8.3), which leaves an error of
+// about 1.5 |M^T M - I|^2 < 2e-12, far below the float rounding of the result; the
+// orthogonal polar factor it converges to is U V^T. Anything else (a reflection, a
+// matrix further from orthogonal) takes projectSO3.
+__device__ __forceinline__ void projectSO3Near(const double* M, double* R) {
+    double G[9];  // M^T M
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) G[i * 3 + j] = M[i] * M[j] + M[3 + i] * M[3 + j] + M[6 + i] * M[6 + j];
+    double e = 0.0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) e = fmax(e, fabs(G[i] - ((i % 4 == 0) ? 1.0 : 0.0)));
+    const double det = M[0] * (M[4] * M[8] - M[5] * M[7]) - M[1] * (M[3] * M[8] - M[5] * M[6]) +
+                       M[2] * (M[3] * M[7] - M[4] * M[6]);
+    if (!(e <= 1e-6) || !(det > 0.0)) {
+        projectSO3(M, R);
+        return;
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const double s0 = ((j == 0) ? 3.0 : 0.0) - G[0 * 3 + j];
+            const double s1 = ((j == 1) ? 3.0 : 0.0) - G[1 * 3 + j];
+            const double s2 = ((j == 2) ? 3.0 : 0.0) - G[2 * 3 + j];
+            R[i * 3 + j] = 0.5 * (M[i * 3 + 0] * s0 + M[i * 3 + 1] * s1 + M[i * 3 + 2] * s2);
+        }
+}
+
 // The pose terms of a pose (column-major 4 x 4) and the inverse of the reference pose.
 __device__ PoseDerived derivePose(const float* pose, const Rigid& rinv, int iterations) {
     const Rigid P = rigidFromPtr(pose);
@@ -418,43 +473,23 @@ __device__ bool icpSolve(IcpState* st, bool writer, const double* sums, const So
     float sq = 0.0f;
     for (int i = 0; i < 6; ++i) sq += x[i] * x[i];
     const float step_norm = sqrtf(sq);
-    const float rx = x[3], ry = x[4], rz = x[5];
-    float Rd[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
-    const float angle = sqrtf(rx * rx + ry * ry + rz * rz);
-    if (angle > 1e-4f) {
-        // Rodrigues' formula R = cI + s[k]x + (1 - c) k k^T for the unit axis k.
-        const float nn = sqrtf(sum3(rx * rx, ry * ry, rz * rz));
-        const float kx = rx / nn, ky = ry / nn, kz = rz / nn;
-        const float s = sinf(angle), co = cosf(angle), oc = 1.0f - co;
-        Rd[0][0] = oc * kx * kx + co;
-        Rd[1][1] = oc * ky * ky + co;
-        Rd[2][2] = oc * kz * kz + co;
-        Rd[0][1] = oc * kx * ky - s * kz;
-        Rd[1][0] = oc * kx * ky + s * kz;
-        Rd[0][2] = oc * kx * kz + s * ky;
-        Rd[2][0] = oc * kx * kz - s * ky;
-        Rd[1][2] = oc * ky * kz - s * kx;
-        Rd[2][1] = oc * ky * kz + s * kx;
-    } else {
-        Rd[0][1] = -rz;
-        Rd[0][2] = ry;
-        Rd[1][0] = rz;
-        Rd[1][2] = -rx;
-        Rd[2][0] = -ry;
-        Rd[2][1] = rx;
-    }
+    // pose <- pose * [R(w) | t] (ICPTracker.cpp:100-121): R(w) by Rodrigues' formula for
+    // |w| > 1e-4 and the first-order I + [w]x below, as the reference's AngleAxis branch
+    // does; the rotation product and its projection in FP64.
+    const double rx = x[3], ry = x[4], rz = x[5];
+    double Rd[9] = {1.0, -rz, ry, rz, 1.0, -rx, -ry, rx, 1.0};
+    if (rx * rx + ry * ry + rz * rz > 1e-8) rodrigues64(rx, ry, rz, Rd);
     // The pose the solved iteration started from.
-    float P[3][3], t[3] = {pd.t_cw.x, pd.t_cw.y, pd.t_cw.z};
-    for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) P[i][j] = pd.R_cw.m[i][j];
+    const float t[3] = {pd.t_cw.x, pd.t_cw.y, pd.t_cw.z};
     double Md[9];
     float nt[3];
     for (int i = 0; i < 3; ++i) {
-        for (int j = 0; j < 3; ++j) Md[i * 3 + j] = (double)((P[i][0] * Rd[0][j] + P[i][1] * Rd[1][j]) + P[i][2] * Rd[2][j]);
-        nt[i] = ((P[i][0] * x[0] + P[i][1] * x[1]) + P[i][2] * x[2]) + t[i];
+        const double p0 = pd.R_cw.m[i][0], p1 = pd.R_cw.m[i][1], p2 = pd.R_cw.m[i][2];
+        for (int j = 0; j < 3; ++j) Md[i * 3 + j] = p0 * Rd[j] + p1 * Rd[3 + j] + p2 * Rd[6 + j];
+        nt[i] = ((pd.R_cw.m[i][0] * x[0] + pd.R_cw.m[i][1] * x[1]) + pd.R_cw.m[i][2] * x[2]) + t[i];
     }
     double Rp[9];
-    projectSO3(Md, Rp);
+    projectSO3Near(Md, Rp);
     float np[16] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1};
     for (int i = 0; i < 3; ++i) {
         for (int j = 0; j < 3; ++j) np[j * 4 + i] = (float)Rp[i * 3 + j];

@@ -8,7 +8,10 @@ full size, against the oracle:
   oracle's, and every frame's ICP from the previous oracle pose lands on the oracle's
   pose by the ICP parity rule (tests/gpu_util.py: 1e-5 m / 1e-5 rad when both converge
   with the same steps per level, 5e-5 when the discrete stop test or the iteration
-  budget separates them);
+  budget separates them), or within the oracle's own spread over summation orders on
+  that frame when that is larger (tests/golden/c2_512_icp_spread.npz: what the
+  reference gives with 1..64 OpenMP threads; above 1e-5 on 6 of the 99 frames, up to
+  1.6e-4 rad);
 * the FP16 + UInt16 form's error statement against the FP64 oracle (TSDF over the
   voxels both observe, raycast depth over the pixels both hit), next to the FP32 form's
   and the storage-only error (FP16 against the FP32 oracle).
@@ -28,6 +31,7 @@ from tests.gpu_util import icp_pose_parity, say, steps_per_level
 pytestmark = pytest.mark.gpu
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "c2_512_poses.npz")
+SPREAD = os.path.join(os.path.dirname(__file__), "golden", "c2_512_icp_spread.npz")
 
 
 @pytest.fixture(scope="module")
@@ -83,6 +87,7 @@ def test_512_bit_exact_first_frames(c2):
 
 def test_512_open_loop_golden(c2):
     g, frames, K, p = c2
+    spread = np.load(SPREAD)["spread"]
     P = g["fp32"].astype(np.float32)
     vol = TSDFVolume(TSDFParams(**p), fmt=N.FMT_F32)
     tr = ICPTracker(640, 480, K, ICPParams(min_depth=p["min_depth"], max_depth=p["max_depth"]))
@@ -98,6 +103,8 @@ def test_512_open_loop_golden(c2):
             ref_steps, ref_conv = int(g["fp32_icp"][k, 3]), bool(g["fp32_icp"][k, 0] > 0)
             ok, dt, ang, same = icp_pose_parity(r["pose"], r["iterations"], P[k], ref_steps,
                                                 r["converged"] and ref_conv)
+            # or within the reference's own summation-order spread on this frame
+            ok = ok or (dt <= spread[k, 0] and ang <= spread[k, 1])
             rows.append((k, ok, dt, ang, same, r["converged"], ref_conv))
         vol.counters(reset=True)
         N.check(N.lib.azu_integrate(vol.h, tr.level(0)[0], 640, 480, N.fptr(Kc), N.fptr(pose_cm(P[k])),
@@ -112,8 +119,9 @@ def test_512_open_loop_golden(c2):
         "%.2e m %.2e rad" % (len(strict), max(r[2] for r in strict), max(r[3] for r in strict),
                              max(r[2] for r in rows), max(r[3] for r in rows)))
     for r in rows:
-        if not (r[4] and r[5] and r[6]):
-            say("  frame %d: %.2e m %.2e rad, same steps %s, converged gpu %s oracle %s" % (r[0:1] + r[2:]))
+        if not (r[4] and r[5] and r[6]) or max(r[2], r[3]) > 1e-5:
+            say("  frame %d: %.2e m %.2e rad, same steps %s, converged gpu %s oracle %s; oracle spread %.2e m %.2e rad"
+                % (r[0:1] + r[2:] + tuple(spread[r[0]])))
     assert len(strict) >= 90
     assert all(r[1] for r in rows), [r for r in rows if not r[1]]
 
