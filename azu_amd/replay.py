@@ -119,16 +119,17 @@ class FrameTraceWriter:
 
 
 def replay(rec_dir, out_dir, volume_params, K, pose0=None, fmt=N.FMT_F16_U16, max_frames=None,
-           motion_model="constant_velocity"):
+           motion_model="constant_velocity", degeneracy_rel=5e-3):
     """Replays the recording's depth frames in order through GradedFrameLoop (raw codes
-    in), writing <out_dir>/trace.csv. Returns the per-frame decision records and the
-    tracked poses (row-major)."""
+    in), writing <out_dir>/trace.csv. degeneracy_rel is keepObservableMotion's ratio (the
+    pipeline's default 5e-3; 0 turns the filter off). Returns the per-frame decision
+    records and the tracked poses (row-major)."""
     os.makedirs(out_dir, exist_ok=True)
     entries = read_index(rec_dir)
     if max_frames is not None:
         entries = entries[:max_frames]
     loop = GradedFrameLoop(volume_params, fmt, WIDTH, HEIGHT, K, motion_model=motion_model,
-                           max_frames=max(1, len(entries)))
+                           max_frames=max(1, len(entries)), degeneracy_rel=degeneracy_rel)
     loop.reset(np.eye(4, dtype=np.float32) if pose0 is None else pose0)
     trace = FrameTraceWriter(os.path.join(out_dir, "trace.csv"))
     buf = N.DeviceBuffer(WIDTH * HEIGHT * 2)

@@ -38,6 +38,8 @@ for sfx, T in (("f32", F), ("f64", D)):
     getattr(lib, "orc_icp_track_" + sfx).argtypes = [P] * 6 + [I, I, P, P, I, I, P, P, P, P, F, F, P, P, P, P]
     getattr(lib, "orc_run_sequence_" + sfx).restype = I
     getattr(lib, "orc_run_sequence_" + sfx).argtypes = [P, P, I, I, I, P, P, P, F, F, P, P, P, P, I, P]
+    getattr(lib, "orc_run_sequence_m_" + sfx).restype = I
+    getattr(lib, "orc_run_sequence_m_" + sfx).argtypes = [P, P, I, I, I, P, P, P, F, F, P, P, I, P]
 lib.orc_jacobi_eigenvalues.argtypes = [P, I, P]
 lib.orc_ldlt_solve_f32.restype = I
 lib.orc_ldlt_solve_f32.argtypes = [P, P, P]
@@ -148,6 +150,22 @@ class Volume:
                                 1 if velocity else 0, _d(icp))
         self.last_icp = icp
         return np.stack([rm(p) for p in poses_out]), ms, nupd
+
+    def run_sequence_m(self, frames_m, K, pose0, iters=(10, 5, 4), dist=0.1, angle=30.0, velocity=False):
+        """The same loop over float metre frames (F, H, W), used as given (already
+        band-gated by the caller's decoding). Returns (poses (F, 4, 4), updated-voxel
+        counts, ICP outcome per frame: converged, final step, inliers, packed steps)."""
+        fr = np.ascontiguousarray(frames_m, np.float32)
+        F_, H, W = fr.shape
+        Kc = np.asarray(K, np.float32)
+        it = np.asarray(iters, np.int32)
+        p0 = cm(pose0)
+        poses_out = np.empty((F_, 16), np.float32)
+        nupd = np.empty(F_, np.int64)
+        icp = np.zeros((F_, 4), np.float64)
+        self.fn("run_sequence_m")(self.h, _d(fr), F_, W, H, _d(Kc), _d(p0), _d(it), dist, angle, _d(poses_out),
+                                  _d(nupd), 1 if velocity else 0, _d(icp))
+        return np.stack([rm(p) for p in poses_out]), nupd, icp
 
     def close(self):
         if self.h:

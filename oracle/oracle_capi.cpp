@@ -62,6 +62,78 @@ template <class T> struct Pyr {
         for (int l = 0; l < 3; ++l) out[l] = Level<T>{v[l].data(), n[l].data(), W[l], H[l]};
     }
 };
+
+// The benchmark frame loop (the pipeline order): frame 0 integrates at pose0 and
+// raycasts; frame k >= 1 builds the live pyramid, tracks against the model raycast at the
+// previous tracked pose (ref = that pose), integrates and raycasts at the tracked pose
+// (velocity != 0: the estimate is the constant-velocity prediction, else the previous
+// pose). Frames are uint16 mm (frames_mm, band-gated here) or float metres (frames_m, as
+// given). poses_out: 16 floats per frame; ms_out: 5 doubles per frame {pyramid, icp,
+// integrate, raycast, total}; with poses_in, frames k >= 1 use poses_in[k] instead of ICP.
+template <class T>
+int runSequence(Volume<T>& V, const uint16_t* frames_mm, const float* frames_m, int nframes, int W, int H,
+                const float* K, const float* pose0, const int* iters, float dist, float angle, const float* poses_in,
+                float* poses_out, double* ms_out, long long* n_upd_out, int velocity, double* icp_out) {
+    const IcpParams prm = mkIcp(iters, dist, angle);
+    const Intr KK = mkK(K);
+    const size_t npx = static_cast<size_t>(W) * H;
+    std::vector<T> depth(npx);
+    std::vector<V3<T>> mv(npx), mn(npx);
+    Pose<T> pose = poseFrom<T>(pose0);
+    Pose<T> last = pose;
+    using clk = std::chrono::steady_clock;
+    auto ms = [](clk::time_point a, clk::time_point b) {
+        return std::chrono::duration<double, std::milli>(b - a).count();
+    };
+    const T mind = static_cast<T>(V.p.min_depth), maxd = static_cast<T>(V.p.max_depth);
+    for (int f = 0; f < nframes; ++f) {
+        const auto t0 = clk::now();
+        if (frames_mm)
+            depthMmToMeters<T>(frames_mm + npx * f, static_cast<int>(npx), mind, maxd, depth.data());
+        else
+            for (size_t i = 0; i < npx; ++i) depth[i] = static_cast<T>(frames_m[npx * f + i]);
+        auto t1 = t0, t2 = t0;
+        if (f > 0) {
+            if (poses_in) {
+                pose = poseFrom<T>(poses_in + 16 * f);
+                t1 = t2 = clk::now();
+            } else {
+                Pyr<T> P;
+                P.build(depth.data(), W, H, KK);
+                Level<T> L[3];
+                P.levels(L);
+                t1 = clk::now();
+                const Pose<T> est = velocity ? predictPose(pose, last) : pose;
+                last = pose;
+                const IcpResult<T> r = track(L, mv.data(), mn.data(), W, H, KK, est, pose, prm);
+                if (icp_out) {
+                    icp_out[4 * f + 0] = r.converged;
+                    icp_out[4 * f + 1] = static_cast<double>(r.final_step);
+                    icp_out[4 * f + 2] = r.inliers;
+                    icp_out[4 * f + 3] = r.steps;
+                }
+                pose = poseFrom<T>(r.pose);
+                t2 = clk::now();
+            }
+        } else {
+            t1 = t2 = clk::now();
+        }
+        const long long nu = integrate(V, depth.data(), W, H, KK, pose, mind, maxd);
+        const auto t3 = clk::now();
+        raycast(V, pose, KK, W, H, mv.data(), mn.data());
+        const auto t4 = clk::now();
+        if (poses_out) poseTo(pose, poses_out + 16 * f);
+        if (n_upd_out) n_upd_out[f] = nu;
+        if (ms_out) {
+            ms_out[5 * f + 0] = ms(t0, t1);
+            ms_out[5 * f + 1] = ms(t1, t2);
+            ms_out[5 * f + 2] = ms(t2, t3);
+            ms_out[5 * f + 3] = ms(t3, t4);
+            ms_out[5 * f + 4] = ms(t0, t4);
+        }
+    }
+    return nframes;
+}
 }  // namespace
 
 #define ORC_API extern "C" __attribute__((visibility("default")))
@@ -168,74 +240,20 @@ template <class T> struct Pyr {
         res_ints[6] = r.dist_filtered; res_ints[7] = r.angle_filtered; res_ints[8] = r.steps;                \
         for (int i = 0; i < 36; ++i) res_info[i] = r.information[i];                                         \
     }                                                                                                        \
-    /* The benchmark frame loop over a uint16 mm sequence (the pipeline order): frame 0 integrates at the    \
-       given pose and raycasts; frame k >= 1 builds the live pyramid, tracks against the model raycast at    \
-       the previous tracked pose (ref = that pose), integrates                                              \
-       and raycasts at the tracked pose (velocity != 0: the estimate is the constant-velocity prediction,  \
-       else the previous pose). poses_out: 16 floats per frame. ms_out: 5 doubles per frame {pyramid, icp, integrate,   \
-       raycast, total}. When poses_in is given, frames k >= 1 use poses_in[k] instead of ICP (integrate +    \
-       raycast only). Returns the number of frames run. */                                                   \
+    /* runSequence over uint16 mm frames, or over float metre frames (_m). */                            \
     ORC_API int orc_run_sequence_##SFX(void* h, const uint16_t* frames_mm, int nframes, int W, int H,        \
                                        const float* K, const float* pose0, const int* iters, float dist,     \
                                        float angle, const float* poses_in, float* poses_out, double* ms_out,  \
                                        long long* n_upd_out, int velocity, double* icp_out) {                \
-        auto& V = *static_cast<Volume<T>*>(h);                                                               \
-        const IcpParams prm = mkIcp(iters, dist, angle);                                                     \
-        const Intr KK = mkK(K);                                                                              \
-        const size_t npx = static_cast<size_t>(W) * H;                                                       \
-        std::vector<T> depth(npx);                                                                           \
-        std::vector<V3<T>> mv(npx), mn(npx);                                                                 \
-        Pose<T> pose = poseFrom<T>(pose0);                                                                   \
-        Pose<T> last = pose;                                                                                 \
-        using clk = std::chrono::steady_clock;                                                               \
-        auto ms = [](clk::time_point a, clk::time_point b) {                                                 \
-            return std::chrono::duration<double, std::milli>(b - a).count();                                \
-        };                                                                                                   \
-        const T mind = static_cast<T>(V.p.min_depth), maxd = static_cast<T>(V.p.max_depth);                  \
-        for (int f = 0; f < nframes; ++f) {                                                                  \
-            const auto t0 = clk::now();                                                                      \
-            depthMmToMeters<T>(frames_mm + npx * f, static_cast<int>(npx), mind, maxd, depth.data());        \
-            auto t1 = t0, t2 = t0;                                                                           \
-            if (f > 0) {                                                                                     \
-                if (poses_in) {                                                                              \
-                    pose = poseFrom<T>(poses_in + 16 * f);                                                   \
-                    t1 = t2 = clk::now();                                                                    \
-                } else {                                                                                     \
-                    Pyr<T> P;                                                                                \
-                    P.build(depth.data(), W, H, KK);                                                         \
-                    Level<T> L[3];                                                                           \
-                    P.levels(L);                                                                             \
-                    t1 = clk::now();                                                                         \
-                    const Pose<T> est = velocity ? predictPose(pose, last) : pose;                           \
-                    last = pose;                                                                             \
-                    const IcpResult<T> r = track(L, mv.data(), mn.data(), W, H, KK, est, pose, prm);         \
-                    if (icp_out) {                                                                           \
-                        icp_out[4 * f + 0] = r.converged;                                                    \
-                        icp_out[4 * f + 1] = static_cast<double>(r.final_step);                              \
-                        icp_out[4 * f + 2] = r.inliers;                                                      \
-                        icp_out[4 * f + 3] = r.steps;                                                        \
-                    }                                                                                        \
-                    pose = poseFrom<T>(r.pose);                                                              \
-                    t2 = clk::now();                                                                         \
-                }                                                                                            \
-            } else {                                                                                         \
-                t1 = t2 = clk::now();                                                                        \
-            }                                                                                                \
-            const long long nu = integrate(V, depth.data(), W, H, KK, pose, mind, maxd);                     \
-            const auto t3 = clk::now();                                                                      \
-            raycast(V, pose, KK, W, H, mv.data(), mn.data());                                                \
-            const auto t4 = clk::now();                                                                      \
-            if (poses_out) poseTo(pose, poses_out + 16 * f);                                                 \
-            if (n_upd_out) n_upd_out[f] = nu;                                                                \
-            if (ms_out) {                                                                                    \
-                ms_out[5 * f + 0] = ms(t0, t1);                                                              \
-                ms_out[5 * f + 1] = ms(t1, t2);                                                              \
-                ms_out[5 * f + 2] = ms(t2, t3);                                                              \
-                ms_out[5 * f + 3] = ms(t3, t4);                                                              \
-                ms_out[5 * f + 4] = ms(t0, t4);                                                              \
-            }                                                                                                \
-        }                                                                                                    \
-        return nframes;                                                                                      \
+        return runSequence(*static_cast<Volume<T>*>(h), frames_mm, nullptr, nframes, W, H, K, pose0, iters,   \
+                           dist, angle, poses_in, poses_out, ms_out, n_upd_out, velocity, icp_out);          \
+    }                                                                                                        \
+    ORC_API int orc_run_sequence_m_##SFX(void* h, const float* frames_m, int nframes, int W, int H,          \
+                                         const float* K, const float* pose0, const int* iters, float dist,   \
+                                         float angle, float* poses_out, long long* n_upd_out, int velocity,  \
+                                         double* icp_out) {                                                  \
+        return runSequence(*static_cast<Volume<T>*>(h), nullptr, frames_m, nframes, W, H, K, pose0, iters,    \
+                           dist, angle, nullptr, poses_out, nullptr, n_upd_out, velocity, icp_out);          \
     }                                                                                                        \
     ORC_API void orc_model_get_##SFX(void* h, const float* pose, const float* K, int W, int H, T* verts,     \
                                      T* norms) {                                                             \

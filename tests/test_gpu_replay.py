@@ -40,7 +40,7 @@ def test_replay_recording(tmp_path):
     assert all(r["quality"] == N.TRACK_GOOD for r in records)
     err = max(pose_diff(poses[k], truth[k])[0] for k in range(nf))
     say("  all Good, max |t - t_true| %.2f mm" % (1000 * err))
-    assert err < 1e-2
+    assert err < 2.5e-3  # 1.96 mm measured (FP16 + U16, the reference's degeneracy filter on)
     rows = list(csv.DictReader(open(tmp_path / "out" / "trace.csv")))
     track = [r for r in rows if r["kind"] == "track"]
     integ = [r for r in rows if r["kind"] == "integ"]
