@@ -21,7 +21,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 
-HIP_SOURCES = ["capi.hip", "integrate.hip", "raycast.hip", "frame.hip", "icp.hip", "adjoint.hip", "mc.hip"]
+HIP_SOURCES = ["capi.hip", "integrate.hip", "raycast.hip", "frame.hip", "icp.hip", "adjoint.hip", "mc.hip", "condition.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("AZU_OFFLOAD_ARCH", "gfx950")
 HIP_FLAGS = [

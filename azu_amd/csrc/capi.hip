@@ -64,6 +64,9 @@ hipError_t launchDownsample(int W, int H, float* const d[3], float* const v[3], 
 hipError_t launchVolConvert(int format, bool to_float, void* vox, float* tsdf, float* weight, size_t n,
                             hipStream_t st);
 hipError_t launchCountObserved(int format, const void* vox, size_t n, unsigned long long* out, hipStream_t st);
+hipError_t launchCondStage(int stage, const uint16_t* in, const float* luma, uint16_t* out, float* state, int W,
+                           int H, float lo, float hi, hipStream_t st);
+hipError_t launchCondZero(float* p, size_t n, hipStream_t st);
 }  // namespace azu
 
 using namespace azu;
@@ -257,6 +260,14 @@ struct azu_volume {
     unsigned* dsafe = nullptr;  // skip bounds, two words per 2 x 2 pixel cell, then one flag word
     size_t dsafe_cells = 0;
     bool exact() const { return mode != AZU_MODE_FAST; }
+};
+
+// Depth conditioner: two ping-pong raw frames for the stage chain and the EMA history.
+struct azu_conditioner {
+    int W = 0, H = 0;
+    int minimal = 0;
+    uint16_t* buf[2] = {nullptr, nullptr};
+    float* ema = nullptr;
 };
 
 struct azu_tracker {
@@ -1175,6 +1186,106 @@ int azu_tracker_phase_times(azu_tracker* t, double out[5], int reset) {
         AZU_HIP_TRY(launchIcpClearTicks(t->state, nullptr), "tick clear launch");
         AZU_HIP_TRY(hipDeviceSynchronize(), "tick clear");
     }
+    return AZU_OK;
+}
+
+// ---- depth conditioning ----------------------------------------------------------
+int azu_cond_destroy(azu_conditioner* c) {
+    if (!c) return AZU_OK;
+    if (c->buf[0]) (void)hipFree(c->buf[0]);
+    if (c->buf[1]) (void)hipFree(c->buf[1]);
+    if (c->ema) (void)hipFree(c->ema);
+    delete c;
+    return AZU_OK;
+}
+
+int azu_cond_create(int W, int H, azu_conditioner** out) {
+    if (!out) return fail(AZU_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    if (W < 4 || H < 4 || W > 8192 || H > 8192) return fail(AZU_ERR_ARG, "depth frame size must be in [4, 8192]");
+    auto* c = new azu_conditioner();
+    c->W = W;
+    c->H = H;
+    const size_t n = (size_t)W * H;
+    const bool ok = hipMalloc(&c->buf[0], n * 2) == hipSuccess && hipMalloc(&c->buf[1], n * 2) == hipSuccess &&
+                    hipMalloc(&c->ema, n * 4) == hipSuccess;
+    if (!ok) {
+        azu_cond_destroy(c);
+        return fail(AZU_ERR_ALLOC, "conditioner allocation failed");
+    }
+    if (launchCondZero(c->ema, n, nullptr) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+        azu_cond_destroy(c);
+        return fail(AZU_ERR_HIP, "conditioner initialisation failed");
+    }
+    *out = c;
+    return AZU_OK;
+}
+
+int azu_cond_reset_ema(azu_conditioner* c, azu_stream s) {
+    if (!c) return fail(AZU_ERR_ARG, "null conditioner");
+    AZU_HIP_TRY(launchCondZero(c->ema, (size_t)c->W * c->H, (hipStream_t)s), "EMA reset launch");
+    return AZU_OK;
+}
+
+int azu_cond_set_minimal_depth(azu_conditioner* c, int minimal) {
+    if (!c) return fail(AZU_ERR_ARG, "null conditioner");
+    c->minimal = minimal != 0;
+    return AZU_OK;
+}
+
+int azu_cond_run_stages(azu_conditioner* c, const uint16_t* d_in, const float* d_luma, uint16_t* d_out, float min_m,
+                        float max_m, int stages, azu_stream s) {
+    if (!c) return fail(AZU_ERR_ARG, "null conditioner");
+    if (!d_in || !d_out) return fail(AZU_ERR_ARG, "null depth buffer");
+    if (stages & ~AZU_COND_ALL) return fail(AZU_ERR_ARG, "unknown conditioning stage bit");
+    if (std::isnan(min_m) || std::isnan(max_m)) return fail(AZU_ERR_ARG, "depth band must not be NaN");
+    const hipStream_t st = (hipStream_t)s;
+    const size_t n = (size_t)c->W * c->H;
+    const int order[4] = {AZU_COND_MEDIAN, AZU_COND_HOLES, AZU_COND_GUIDED, AZU_COND_EMA};
+    const char* names[4] = {"cond_median", "cond_holes", "cond_guided", "cond_ema"};
+    int last = -1;
+    for (int k = 0; k < 4; ++k)
+        if (stages & order[k]) last = k;
+    if (last < 0) {
+        if (d_out != d_in) AZU_HIP_TRY(hipMemcpyAsync(d_out, d_in, n * 2, hipMemcpyDeviceToDevice, st), "depth copy");
+        return AZU_OK;
+    }
+    // Every stage reads whole windows of its source, so no stage writes the buffer it
+    // reads: an aliased caller frame is first copied into the handle's own buffers.
+    const uint16_t* src = d_in;
+    if (d_in == d_out) {
+        AZU_HIP_TRY(hipMemcpyAsync(c->buf[1], d_in, n * 2, hipMemcpyDeviceToDevice, st), "depth copy");
+        src = c->buf[1];
+    }
+    for (int k = 0; k <= last; ++k) {
+        if (!(stages & order[k])) continue;
+        uint16_t* dst = k == last ? d_out : (src == c->buf[0] ? c->buf[1] : c->buf[0]);
+        ProfScope ps(names[k], st);
+        AZU_HIP_TRY(launchCondStage(order[k], src, d_luma, dst, c->ema, c->W, c->H, min_m, max_m, st),
+                    "conditioning launch");
+        src = dst;
+    }
+    return AZU_OK;
+}
+
+int azu_cond_process(azu_conditioner* c, const uint16_t* d_in, const float* d_luma, uint16_t* d_out, float min_m,
+                     float max_m, azu_stream s) {
+    if (!c) return fail(AZU_ERR_ARG, "null conditioner");
+    return azu_cond_run_stages(c, d_in, d_luma, d_out, min_m, max_m, c->minimal ? AZU_COND_MEDIAN : AZU_COND_ALL,
+                               s);
+}
+
+int azu_cond_read_ema(azu_conditioner* c, float* dst, azu_stream s) {
+    if (!c || !dst) return fail(AZU_ERR_ARG, "null argument");
+    AZU_HIP_TRY(hipMemcpyAsync(dst, c->ema, (size_t)c->W * c->H * 4, hipMemcpyDefault, (hipStream_t)s),
+                "EMA state copy");
+    return AZU_OK;
+}
+
+int azu_cond_write_ema(azu_conditioner* c, const float* src, azu_stream s) {
+    if (!c || !src) return fail(AZU_ERR_ARG, "null argument");
+    AZU_HIP_TRY(hipMemcpyAsync(c->ema, src, (size_t)c->W * c->H * 4, hipMemcpyDefault, (hipStream_t)s),
+                "EMA state copy");
     return AZU_OK;
 }
 

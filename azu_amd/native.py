@@ -151,6 +151,16 @@ _SIGS = {
     "azu_icp_result_default": (ctypes.c_int, [ctypes.POINTER(IcpResult)]),
     "azu_tracker_prepare_mm": (ctypes.c_int, [c_void_p, c_void_p, c_void_p]),
     "azu_tracker_prepare_raw": (ctypes.c_int, [c_void_p, c_void_p, c_void_p]),
+    "azu_cond_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.POINTER(c_void_p)]),
+    "azu_cond_destroy": (ctypes.c_int, [c_void_p]),
+    "azu_cond_reset_ema": (ctypes.c_int, [c_void_p, c_void_p]),
+    "azu_cond_set_minimal_depth": (ctypes.c_int, [c_void_p, ctypes.c_int]),
+    "azu_cond_process": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_float, ctypes.c_float,
+                                        c_void_p]),
+    "azu_cond_run_stages": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_float, ctypes.c_float,
+                                           ctypes.c_int, c_void_p]),
+    "azu_cond_read_ema": (ctypes.c_int, [c_void_p, c_void_p, c_void_p]),
+    "azu_cond_write_ema": (ctypes.c_int, [c_void_p, c_void_p, c_void_p]),
     "azu_tracker_prepare_m": (ctypes.c_int, [c_void_p, c_void_p, c_void_p]),
     "azu_tracker_load_level": (ctypes.c_int, [c_void_p, ctypes.c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "azu_tracker_downsample": (ctypes.c_int, [c_void_p, c_void_p]),

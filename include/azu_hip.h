@@ -334,6 +334,49 @@ int azu_tracker_result(azu_tracker* t, azu_icp_result* out, azu_stream s);
  * Synchronises; reset != 0 clears. */
 int azu_tracker_phase_times(azu_tracker* t, double out[5], int reset);
 
+/* ---- depth conditioning (kfusion::sensor::SignalConditioner, include/sensor/SignalConditioner.h:29-47) ----
+ * The depth half of SignalConditioner::process (src/sensor/SignalConditioner_omp.cpp:
+ * 337-349) over raw Kinect v1 11-bit codes (uint16, W x H, row-major): a 3x3 median
+ * (denoiseDepthSpatial, :442-478), then, unless the handle is in minimal mode, the raw-0
+ * hole fill (fillDepthHoles, :578-635), the 9x9 luma-guided filter (guidedDepthFilter,
+ * :637-716) and the temporal EMA (applyDepthEma, :494-576), whose per-pixel history the
+ * handle keeps. The RGB half (bilateral, median blur, CAS sharpening) is not here: the
+ * guidance luma it produces (SignalConditioner_omp.cpp:380-385, one float in [0, 1] per
+ * pixel) is an input; NULL stands for the all-zero guidance of a cleared conditioner
+ * (:283). Codes are the reference's bit for bit; the guided filter's Gaussians are the
+ * correctly rounded float exponential (the reference calls the C library's expf). */
+typedef struct azu_conditioner azu_conditioner;
+enum {
+    AZU_COND_MEDIAN = 1, /* denoiseDepthSpatial */
+    AZU_COND_HOLES = 2,  /* fillDepthHoles */
+    AZU_COND_GUIDED = 4, /* guidedDepthFilter */
+    AZU_COND_EMA = 8,    /* applyDepthEma */
+    AZU_COND_ALL = 15
+};
+/* SignalConditioner() (SignalConditioner_omp.cpp:233-245) for W x H frames, W, H in
+ * [4, 8192]; the EMA history starts at 0 (no history). Synchronises. */
+int azu_cond_create(int width, int height, azu_conditioner** out);
+int azu_cond_destroy(azu_conditioner* c);
+/* resetEMA() (SignalConditioner_omp.cpp:304-311): every history slot to +0.0f. The
+ * handle holds no other depth state, so this is also reset()'s depth part (:273-302). */
+int azu_cond_reset_ema(azu_conditioner* c, azu_stream s);
+/* setMinimalDepth (SignalConditioner.h:37): process() runs the median only. */
+int azu_cond_set_minimal_depth(azu_conditioner* c, int minimal);
+/* process()'s depth stages in the reference's order, band [min_m, max_m] metres.
+ * d_out may equal d_in. d_luma: W x H floats or NULL. */
+int azu_cond_process(azu_conditioner* c, const uint16_t* d_in, const float* d_luma, uint16_t* d_out,
+                     float min_m, float max_m, azu_stream s);
+/* Any subset of the stages (AZU_COND_* bits), still in the order median, holes,
+ * guided, EMA: the reference's single-stage test seams fillDepthHolesForTests /
+ * applyDepthEmaForTests (SignalConditioner.h:201-206) are AZU_COND_HOLES / AZU_COND_EMA. */
+int azu_cond_run_stages(azu_conditioner* c, const uint16_t* d_in, const float* d_luma, uint16_t* d_out,
+                        float min_m, float max_m, int stages, azu_stream s);
+/* emaStateForTests (SignalConditioner.h:208): the W x H history in metres, 0 = none, to
+ * dst (device or host memory; a host dst is complete once the stream has synchronised),
+ * and the inverse, for callers that restore a conditioner's state. */
+int azu_cond_read_ema(azu_conditioner* c, float* dst, azu_stream s);
+int azu_cond_write_ema(azu_conditioner* c, const float* src, azu_stream s);
+
 /* ---- per-kernel timing (HIP events on the launch stream) ---- */
 /* While enabled, every kernel launch is bracketed by events; azu_profile_read
  * synchronises and returns, per kernel name, launches and total milliseconds.

@@ -56,6 +56,11 @@ lib.orc_set_icp_chunks.argtypes = [I]
 lib.orc_predict_pose_f32.argtypes = [P, P, P]
 lib.orc_predict_pose_f64.argtypes = [P, P, P]
 lib.orc_mc_extract.restype = LL
+lib.orc_cond_median.argtypes = [P, P, I, I, F, F]
+lib.orc_cond_fill_holes.argtypes = [P, P, I, I, F, F]
+lib.orc_cond_guided.argtypes = [P, P, P, I, I, F, F, I]
+lib.orc_cond_ema.argtypes = [P, P, P, I, I, F, F]
+lib.orc_cond_process.argtypes = [P, P, P, P, I, I, F, F, I, I]
 lib.orc_mc_extract.argtypes = [P, P, I, F, P, LL, P, P, P, P, P, P]
 
 
@@ -300,3 +305,55 @@ def depth_raw(raw, min_d=0.3, max_d=5.0):
         m = np.float32(1.0) / (r.astype(np.float32) * np.float32(-0.0030711016) + np.float32(3.3309495161))
     keep = valid & np.isfinite(m) & (m >= np.float32(min_d)) & (m <= np.float32(max_d))
     return np.where(keep, m, np.float32(0.0)).astype(np.float32)
+
+
+# Depth conditioning (src/sensor/SignalConditioner_omp.cpp:337-716), restated in
+# oracle_condition.hpp. Stage bits, applied in this order.
+COND_MEDIAN, COND_HOLES, COND_GUIDED, COND_EMA = 1, 2, 4, 8
+COND_FULL = 15
+
+
+def _u16(a):
+    return np.ascontiguousarray(a, np.uint16)
+
+
+def cond_median(raw, lo=0.3, hi=2.5):
+    r = _u16(raw)
+    out = np.empty_like(r)
+    lib.orc_cond_median(_d(r), _d(out), r.shape[1], r.shape[0], lo, hi)
+    return out
+
+
+def cond_fill_holes(raw, lo=0.3, hi=2.5):
+    r = _u16(raw)
+    out = np.empty_like(r)
+    lib.orc_cond_fill_holes(_d(r), _d(out), r.shape[1], r.shape[0], lo, hi)
+    return out
+
+
+def cond_guided(raw, luma=None, lo=0.3, hi=2.5, exp_mode=0):
+    """exp_mode 0: the C library's expf, as the reference; 1: the double exponential
+    rounded to float (correctly rounded in practice)."""
+    r = _u16(raw)
+    lu = np.zeros(r.shape, np.float32) if luma is None else np.ascontiguousarray(luma, np.float32)
+    out = np.empty_like(r)
+    lib.orc_cond_guided(_d(r), _d(lu), _d(out), r.shape[1], r.shape[0], lo, hi, exp_mode)
+    return out
+
+
+def cond_ema(raw, state, lo=0.3, hi=2.5):
+    """One EMA pass; state (float32, metres, 0 = no history) is updated in place."""
+    r = _u16(raw)
+    assert state.dtype == np.float32 and state.flags.c_contiguous and state.shape == r.shape
+    out = np.empty_like(r)
+    lib.orc_cond_ema(_d(r), _d(out), _d(state), r.shape[1], r.shape[0], lo, hi)
+    return out
+
+
+def cond_process(raw, state, luma=None, lo=0.3, hi=2.5, stages=COND_FULL, exp_mode=0):
+    r = _u16(raw)
+    lu = np.zeros(r.shape, np.float32) if luma is None else np.ascontiguousarray(luma, np.float32)
+    assert state.dtype == np.float32 and state.flags.c_contiguous and state.shape == r.shape
+    out = np.empty_like(r)
+    lib.orc_cond_process(_d(r), _d(lu), _d(out), _d(state), r.shape[1], r.shape[0], lo, hi, stages, exp_mode)
+    return out

@@ -5,6 +5,7 @@
 #include <chrono>
 
 #include "oracle.hpp"
+#include "oracle_condition.hpp"
 #include "oracle_mc.hpp"
 
 using namespace orc;
@@ -319,4 +320,24 @@ ORC_API long long orc_mc_extract(const float* tsdf, const float* weight, int res
         cache_key = nullptr;
     }
     return *nvert;
+}
+
+// Depth conditioning stages over raw codes (W x H, row-major). luma may be null for the
+// stages that do not read it; state is the per-pixel EMA history in metres.
+ORC_API void orc_cond_median(const uint16_t* in, uint16_t* out, int W, int H, float lo, float hi) {
+    orc::cond::median(in, out, W, H, lo, hi);
+}
+ORC_API void orc_cond_fill_holes(const uint16_t* in, uint16_t* out, int W, int H, float lo, float hi) {
+    orc::cond::fillHoles(in, out, W, H, lo, hi);
+}
+ORC_API void orc_cond_guided(const uint16_t* in, const float* luma, uint16_t* out, int W, int H, float lo, float hi,
+                             int exp_mode) {
+    orc::cond::guided(in, luma, out, W, H, lo, hi, exp_mode);
+}
+ORC_API void orc_cond_ema(const uint16_t* in, uint16_t* out, float* state, int W, int H, float lo, float hi) {
+    orc::cond::ema(in, out, state, W, H, lo, hi);
+}
+ORC_API void orc_cond_process(const uint16_t* in, const float* luma, uint16_t* out, float* state, int W, int H,
+                              float lo, float hi, int stages, int exp_mode) {
+    orc::cond::process(in, luma, out, state, W, H, lo, hi, stages, exp_mode);
 }
