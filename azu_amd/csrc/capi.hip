@@ -684,7 +684,7 @@ static int integrateImpl(azu_volume* v, const float* d_depth, int W, int H, cons
     ProfScope ps(exact ? (v->format == AZU_FMT_F32 ? "integrate_exact_f32" : "integrate_exact_f16")
                        : (v->format == AZU_FMT_F32 ? "integrate_fast_f32" : "integrate_fast_f16"),
                  (hipStream_t)s);
-    if (exact && !bounds_scratch) {
+    if (!bounds_scratch) {
         const size_t need = depthBoundsBytes(W, H);
         if (need > v->bounds_bytes) {
             // grows only when the frame size does; the old buffer may still be read by
@@ -699,7 +699,7 @@ static int integrateImpl(azu_volume* v, const float* d_depth, int W, int H, cons
     }
     AZU_HIP_TRY(launchIntegrate(v->format, exact, v->vox, toVol(v->params), Intr{K[0], K[1], K[2], K[3]}, W, H,
                                 d_depth, min_d, max_d, pa, v->dtr, v->recip,
-                                exact ? (bounds_scratch ? bounds_scratch : v->bounds) : nullptr, v->counters,
+                                bounds_scratch ? bounds_scratch : v->bounds, v->counters,
                                 v->bricks, (hipStream_t)s),
                 "integrate launch");
     v->frames += 1;

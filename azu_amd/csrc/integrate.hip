@@ -13,10 +13,11 @@
 //    chunk transposed, lane = column, so each read-modify-write covers 32 consecutive
 //    voxels of two rows. Division and square root are IEEE-exact, nothing contracts.
 //
-//  * integrateFast<Store>: the FP16 + UInt16 storage form. One thread owns 4
-//    consecutive voxels of a row (one 16-byte word for FP16/U16), the camera
-//    coordinate is an affine function of x evaluated with fmaf, reciprocals are the
-//    hardware v_rcp_f32. Accuracy is stated against the FP64 oracle.
+//  * integrateFast<Store>: the FP16 + UInt16 storage form. The same tiles and tile
+//    classes, but lane = x column: the camera coordinate is an affine function of
+//    (x, z) evaluated with fmaf and the reciprocals are the hardware v_rcp_f32, so the
+//    lanes are independent voxels and each row of a tile is one coalesced access.
+//    Accuracy is stated against the FP64 oracle.
 //
 // Both read the pose by value or from device memory (the ICP output), so a frame can
 // run with no host synchronisation. Updated / newly observed voxel counts are
@@ -460,125 +461,156 @@ __global__ __launch_bounds__(256) void integrateExact(typename S::Word* __restri
 }
 
 // ---------------------------------------------------------------------------
-// Fast kernel. Block = 256 threads over a tile of 64 x-voxels (16 quads) x 16 rows (y)
-// at one z; grid covers the whole volume, tiles outside the AABB exit at once.
+// Fast kernel (the FP16 + UInt16 storage form; also selectable for FP32). The same
+// tiles and classes as integrateExact (cull / free / per-voxel), but a voxel's camera
+// coordinate is the affine c0 + x ax + z az evaluated with fmaf and the reciprocals
+// are the hardware's, so the lanes of a wave are independent voxels: lane = x column
+// of a 64-column tile, the wave walks kFastRows z rows of it at one y. Free tiles fold
+// tsdf_new = 1 with no projection. tileClass's margins cover the serial-add rounding
+// of the exact kernel, which bounds this kernel's few-ulp affine rounding as well, so
+// every class is the outcome the per-voxel fast path would reach.
 // ---------------------------------------------------------------------------
-template <class S> struct QuadIO;
-template <> struct QuadIO<StoreF16> {
-    using Q = uint4;
-    __device__ static Q load(const uint32_t* p) { return *reinterpret_cast<const uint4*>(p); }
-    __device__ static void store(uint32_t* p, Q q) { *reinterpret_cast<uint4*>(p) = q; }
-    __device__ static uint32_t get(const Q& q, int i) { return i == 0 ? q.x : i == 1 ? q.y : i == 2 ? q.z : q.w; }
-    __device__ static void set(Q& q, int i, uint32_t v) {
-        if (i == 0) q.x = v; else if (i == 1) q.y = v; else if (i == 2) q.z = v; else q.w = v;
-    }
-};
-template <> struct QuadIO<StoreF32> {
-    struct Q {
-        float4 a, b;
-    };
-    __device__ static Q load(const float2* p) {
-        const float4* q = reinterpret_cast<const float4*>(p);
-        return Q{q[0], q[1]};
-    }
-    __device__ static void store(float2* p, const Q& v) {
-        float4* q = reinterpret_cast<float4*>(p);
-        q[0] = v.a;
-        q[1] = v.b;
-    }
-    __device__ static float2 get(const Q& q, int i) {
-        return i == 0 ? make_float2(q.a.x, q.a.y) : i == 1 ? make_float2(q.a.z, q.a.w)
-             : i == 2 ? make_float2(q.b.x, q.b.y) : make_float2(q.b.z, q.b.w);
-    }
-    __device__ static void set(Q& q, int i, float2 v) {
-        if (i == 0) { q.a.x = v.x; q.a.y = v.y; }
-        else if (i == 1) { q.a.z = v.x; q.a.w = v.y; }
-        else if (i == 2) { q.b.x = v.x; q.b.y = v.y; }
-        else { q.b.z = v.x; q.b.w = v.y; }
-    }
-};
+constexpr int kFastRows = 16;  // z rows per tile
+constexpr int kFastBatch = 8;  // rows whose loads are issued together
 
-constexpr int kFastZ = 8;  // z slices per block
+// Brick-map bytes for one row of a 64-column tile at (y, z): mask bit j = column xc + j
+// left a non-far voxel. Column 16 k is also the last layer of x-brick xc / 16 + k - 1;
+// y = 16 q and z = 16 q are also in the y- and z-bricks below.
+__device__ __forceinline__ void markRow(unsigned char* __restrict__ bricks, int nb, int xc, int y, int z,
+                                        unsigned long long mask) {
+    const int by = y / kBrick, bz = z / kBrick, bx0 = xc / kBrick;
+    const int ny = (y % kBrick == 0 && by > 0) ? 2 : 1, nz = (z % kBrick == 0 && bz > 0) ? 2 : 1;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const unsigned sub = (unsigned)(mask >> (16 * k)) & 0xffffu;
+        if (!sub) continue;
+        const int bx = bx0 + k;
+        const int nx = ((sub & 1u) && bx > 0) ? 2 : 1;
+        for (int kz = 0; kz < nz; ++kz)
+            for (int ky = 0; ky < ny; ++ky)
+                for (int kx = 0; kx < nx; ++kx)
+                    if (bx - kx < nb) bricks[((size_t)(bz - kz) * nb + (by - ky)) * nb + (bx - kx)] = 1;
+    }
+}
 
 template <class S>
 __global__ __launch_bounds__(256) void integrateFast(typename S::Word* __restrict__ vox, VolParams p, Intr K, int W,
                                                      int H, const float* __restrict__ depth, float min_d, float max_d,
-                                                     PoseArg pa, unsigned long long* __restrict__ counters,
+                                                     PoseArg pa, DepthBounds bnd,
+                                                     unsigned long long* __restrict__ counters,
                                                      unsigned char* __restrict__ bricks, int nb) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const IntFrame f = integrateSetup(p, pa, K, W, H, max_d);
     if (!f.ok) return;
-    const int tx0 = blockIdx.x * 64, ty0 = blockIdx.y * 16, tz0 = blockIdx.z * kFastZ;
-    if (tx0 > f.x1 || tx0 + 63 < f.x0 || ty0 > f.y1 || ty0 + 15 < f.y0 || tz0 > f.z1 || tz0 + kFastZ - 1 < f.z0)
-        return;
-    const int q = threadIdx.x & 15;
-    const int y = ty0 + (threadIdx.x >> 4);
-    const int xq = tx0 + q * 4;
-    const bool row_ok = y >= f.y0 && y <= f.y1 && y < p.res;
-    const float vs = p.vs;
-    const float cu = K.cx + 0.5f, cv = K.cy + 0.5f;
+    const int ny = f.y1 - f.y0 + 1;
+    const int zbase = f.z0 & ~(kFastRows - 1);
+    const int nzg = (f.z1 - zbase + kFastRows) / kFastRows;
+    const int task = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wv);
+    if (task >= ny * nzg) return;  // whole wave; no block barrier below
+    const int y = f.y0 + task % ny;
+    const int zb = zbase + (task / ny) * kFastRows;
+    const int za = max(zb, f.z0), ze = min(zb + kFastRows - 1, f.z1);
+    const float vs = p.vs, trunc = p.trunc;
     const F3 ax = f.step;
-    // Camera coordinate of voxel (x, y, z): c = base(y, z) + x * ax, and the projection
-    // numerators u_num = fx c.x + (cx + 1/2) c.z, v_num = fy c.y + (cy + 1/2) c.z are
-    // affine in x as well: u = floor(u_num / c.z).
-    const float su = fmaf(K.fx, ax.x, cu * ax.z), sv = fmaf(K.fy, ax.y, cv * ax.z);
     const F3 az{f.R_wc.m[0][2] * vs, f.R_wc.m[1][2] * vs, f.R_wc.m[2][2] * vs};
-    const F3 pw{p.origin[0], p.origin[1] + (float)y * vs, p.origin[2] + (float)tz0 * vs};
-    F3 base = f3add(mulv(f.R_wc, pw), f.t_wc);
-    const float inv_trunc = 1.0f / p.trunc, neg_trunc = -p.trunc;
-    const int zend = min(tz0 + kFastZ - 1, f.z1);
-    int nupd = 0, nnew = 0;
-    for (int z = tz0; z <= zend; ++z, base = f3add(base, az)) {
-        if (z < f.z0) continue;
-        const float bu = fmaf(K.fx, base.x, cu * base.z), bv = fmaf(K.fy, base.y, cv * base.z);
-        float tn[4];
-        bool any = false;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            tn[i] = __builtin_nanf("");
-            const int x = xq + i;
-            if (!row_ok || x < f.x0 || x > f.x1) continue;
-            const float fx = (float)x;
-            const float cz = fmaf(fx, ax.z, base.z);
-            if (!(cz > 0.0f)) continue;
-            const float rz = __builtin_amdgcn_rcpf(cz);
-            const float uf = floorf(fmaf(fx, su, bu) * rz);
-            const float vf = floorf(fmaf(fx, sv, bv) * rz);
-            if (!(uf >= 0.0f && uf < (float)W && vf >= 0.0f && vf < (float)H)) continue;
-            const float d = depth[(int)vf * W + (int)uf];
-            if (!(d >= min_d && d <= max_d)) continue;
-            const float sdf = d - cz;
-            if (sdf < neg_trunc) continue;
-            tn[i] = fminf(1.0f, sdf * inv_trunc);
-            any = true;
+    const F3 c0 = f3add(mulv(f.R_wc, F3{p.origin[0], p.origin[1] + (float)y * vs, p.origin[2]}), f.t_wc);
+    const float cu = K.cx + 0.5f, cv = K.cy + 0.5f;
+    // u = (fx c.x + cu c.z) / c.z, affine numerators in x
+    const float su = fmaf(K.fx, ax.x, cu * ax.z), sv = fmaf(K.fy, ax.y, cv * ax.z);
+    const float inv_trunc = __builtin_amdgcn_rcpf(trunc);
+    const float fW = (float)W, fH = (float)H;
+    const size_t res = (size_t)p.res, plane = res * res;
+    const float cbound = fabsf(f.t_wc.x) + fabsf(f.t_wc.y) + fabsf(f.t_wc.z) + fabsf(p.origin[0]) +
+                         fabsf(p.origin[1]) + fabsf(p.origin[2]) + 3.0f * (float)p.res * vs;
+    unsigned long long n_upd = 0, n_new = 0;  // wave-uniform (ballot counts)
+
+    const int xstart = f.x0 & ~63;
+    unsigned long long cls_lo, cls_hi;
+    {
+        int my = kTileCull;
+        const int xt = xstart + lane * 64;
+        if (xt <= f.x1) {
+            const int js = max(f.x0 - xt, 0), n = min(64, f.x1 - xt + 1);
+            my = tileClass(f, p, K, W, H, max_d, xt + js, xt + n - 1, y, za, ze, bnd, cbound);
         }
-        if (any) {
-            typename S::Word* ptr = vox + ((size_t)z * p.res + (size_t)y) * (size_t)p.res + (size_t)xq;
-            typename QuadIO<S>::Q qv = QuadIO<S>::load(ptr);
+        cls_lo = __ballot((my & 1) != 0);
+        cls_hi = __ballot((my & 2) != 0);
+    }
+    for (int xc = xstart, ti = 0; xc <= f.x1; xc += 64, ++ti) {
+        const int cls = (int)((cls_lo >> ti) & 1ull) | ((int)((cls_hi >> ti) & 1ull) << 1);
+        if (cls == kTileCull) continue;
+        const int x = xc + lane;
+        const bool col_ok = x >= f.x0 && x <= f.x1;
+        const float xf = (float)x;
+        const size_t tile0 = (size_t)za * plane + (size_t)y * res + (size_t)xc;
+        const size_t left = (res * plane - tile0) * sizeof(typename S::Word);
+        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+            vox + tile0, (short)0, (int)(left < 0x80000000ull ? left : 0x80000000ull), 0x00020000);
+        const unsigned lane_off = (unsigned)lane * (unsigned)sizeof(typename S::Word);
+        const unsigned row_step = (unsigned)(plane * sizeof(typename S::Word));
+#pragma unroll 1
+        for (int z0 = za; z0 <= ze; z0 += kFastBatch) {
+            float tn[kFastBatch];
+            if (cls == kTileFree) {
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                if (__builtin_isnan(tn[i])) continue;
-                const typename S::Word wd = QuadIO<S>::get(qv, i);
-                const float w_old = S::weight(wd);
-                const float t_old = S::tsdf(wd);
-                const float t_new = fmaf(t_old, w_old, tn[i]) * __builtin_amdgcn_rcpf(w_old + 1.0f);
-                const typename S::Word nw = S::pack(t_new, fminf(w_old + 1.0f, p.max_w));
-                QuadIO<S>::set(qv, i, nw);
-                if (bricks && !S::far(nw)) markBricks(bricks, nb, xq + i, y, z);
-                ++nupd;
-                nnew += (w_old == 0.0f);
+                for (int k = 0; k < kFastBatch; ++k) tn[k] = (col_ok && z0 + k <= ze) ? 1.0f : __builtin_nanf("");
+            } else {
+                float cz[kFastBatch], dj[kFastBatch];
+                int pix[kFastBatch];
+#pragma unroll
+                for (int k = 0; k < kFastBatch; ++k) {
+                    pix[k] = -1;
+                    const float zf = (float)(z0 + k);
+                    const F3 cr{fmaf(zf, az.x, c0.x), fmaf(zf, az.y, c0.y), fmaf(zf, az.z, c0.z)};
+                    cz[k] = fmaf(xf, ax.z, cr.z);
+                    if (col_ok && z0 + k <= ze && cz[k] > 0.0f) {
+                        const float bu = fmaf(K.fx, cr.x, cu * cr.z), bv = fmaf(K.fy, cr.y, cv * cr.z);
+                        const float rz = __builtin_amdgcn_rcpf(cz[k]);
+                        const float uf = floorf(fmaf(xf, su, bu) * rz);
+                        const float vf = floorf(fmaf(xf, sv, bv) * rz);
+                        if (uf >= 0.0f && uf < fW && vf >= 0.0f && vf < fH) pix[k] = (int)vf * W + (int)uf;
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < kFastBatch; ++k) dj[k] = depth[pix[k] < 0 ? 0 : pix[k]];
+#pragma unroll
+                for (int k = 0; k < kFastBatch; ++k) {
+                    tn[k] = __builtin_nanf("");
+                    const float d = dj[k];
+                    if (pix[k] >= 0 && d >= min_d && d <= max_d) {
+                        const float sdf = d - cz[k];
+                        if (!(sdf < -trunc)) tn[k] = fminf(1.0f, sdf * inv_trunc);
+                    }
+                }
             }
-            QuadIO<S>::store(ptr, qv);
-        }
-    }
-    if (counters) {
-        unsigned long long a = (unsigned long long)nupd, b = (unsigned long long)nnew;
+            typename S::Word wd[kFastBatch];
+            const unsigned soff0 = (unsigned)(z0 - za) * row_step;
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            a += __shfl_xor(a, o);
-            b += __shfl_xor(b, o);
+            for (int k = 0; k < kFastBatch; ++k)
+                wd[k] = BufWord<S>::load(rs, __builtin_isnan(tn[k]) ? 0x80000000u : lane_off, soff0 + k * row_step);
+#pragma unroll
+            for (int k = 0; k < kFastBatch; ++k) {
+                const bool upd = !__builtin_isnan(tn[k]);
+                bool nf = false, newly = false;
+                if (upd) {
+                    const float w_old = S::weight(wd[k]);
+                    const float t_old = S::tsdf(wd[k]);
+                    newly = w_old == 0.0f;
+                    const float t_new = fmaf(t_old, w_old, tn[k]) * __builtin_amdgcn_rcpf(w_old + 1.0f);
+                    const typename S::Word nw = S::pack(t_new, fminf(w_old + 1.0f, p.max_w));
+                    BufWord<S>::store(nw, rs, lane_off, soff0 + k * row_step);
+                    nf = !S::far(nw);
+                }
+                n_upd += __builtin_popcountll(__ballot(upd));
+                n_new += __builtin_popcountll(__ballot(newly));
+                if (bricks) {
+                    const unsigned long long m = __ballot(nf);
+                    if (m && lane == 0) markRow(bricks, nb, xc, y, z0 + k, m);
+                }
+            }
         }
-        flushCounts(counters, a, b);
     }
+    if (counters) flushCounts(counters, n_upd, n_new);
 }
 
 // Exhaustive comparison of a fast form with the IEEE result over every float whose
@@ -687,6 +719,25 @@ size_t depthBoundsBytes(int W, int H) {
 // dtr: division by the truncation, verified exact (dtr.fast) or IEEE; recip: 1 / c.z by
 // the verified Newton-Raphson reciprocal; bounds: scratch for the depth bounds
 // (depthBoundsBytes), or null to decide every tile voxel by voxel.
+// The depth-bounds pyramid of the frame into `bounds` (null: no bounds, every
+// in-frustum tile is decided voxel by voxel).
+static DepthBounds boundsFor(void* bounds, const float* depth, int W, int H, float min_d, float max_d,
+                             hipStream_t st) {
+    DepthBounds bnd{};
+    if (!bounds) return bnd;
+    int off = 0;
+    for (int l = 1; l <= kBoundLevels; ++l) {
+        bnd.off[l] = off;
+        bnd.w[l] = (W + (1 << l) - 1) >> l;
+        bnd.h[l] = (H + (1 << l) - 1) >> l;
+        off += bnd.w[l] * bnd.h[l];
+    }
+    bnd.lvl = static_cast<const float2*>(bounds);
+    hipLaunchKernelGGL(depthBoundsKernel, dim3((W + 63) / 64, (H + 63) / 64), dim3(256), 0, st, depth, W, H, min_d,
+                       max_d, static_cast<float2*>(bounds), bnd);
+    return bnd;
+}
+
 // bricks: the brick map to keep (brickCount(res)^3 bytes), or null.
 hipError_t launchIntegrate(int format, bool exact, void* vox, const VolParams& p, Intr K, int W, int H,
                            const float* depth, float min_d, float max_d, const PoseArg& pa, const FixedDiv& dtr,
@@ -694,19 +745,7 @@ hipError_t launchIntegrate(int format, bool exact, void* vox, const VolParams& p
                            hipStream_t st) {
     const int nb = brickCount(p.res);
     if (exact) {
-        DepthBounds bnd{};
-        if (bounds) {
-            int off = 0;
-            for (int l = 1; l <= kBoundLevels; ++l) {
-                bnd.off[l] = off;
-                bnd.w[l] = (W + (1 << l) - 1) >> l;
-                bnd.h[l] = (H + (1 << l) - 1) >> l;
-                off += bnd.w[l] * bnd.h[l];
-            }
-            bnd.lvl = static_cast<const float2*>(bounds);
-            hipLaunchKernelGGL(depthBoundsKernel, dim3((W + 63) / 64, (H + 63) / 64), dim3(256), 0, st, depth, W, H,
-                               min_d, max_d, static_cast<float2*>(bounds), bnd);
-        }
+        const DepthBounds bnd = boundsFor(bounds, depth, W, H, min_d, max_d, st);
         // One wave per (y, 64-row z group) of the whole volume; waves outside the
         // frustum box exit at once (the box is only known on the device).
         const long long tasks = (long long)p.res * ((p.res + 63) / 64);
@@ -731,13 +770,17 @@ hipError_t launchIntegrate(int format, bool exact, void* vox, const VolParams& p
         }
 #undef AZU_INT_EXACT
     } else {
-        const dim3 grid((p.res + 63) / 64, (p.res + 15) / 16, (p.res + kFastZ - 1) / kFastZ);
+        const DepthBounds bnd = boundsFor(bounds, depth, W, H, min_d, max_d, st);
+        // One wave per (y, kFastRows-row z group) of the whole volume; waves outside the
+        // frustum box exit at once.
+        const long long tasks = (long long)p.res * ((p.res + kFastRows - 1) / kFastRows + 1);
+        const dim3 grid((unsigned)((tasks + 3) / 4));
         if (format == kFmtF32)
             hipLaunchKernelGGL(integrateFast<StoreF32>, grid, dim3(256), 0, st, (float2*)vox, p, K, W, H, depth,
-                               min_d, max_d, pa, counters, bricks, nb);
+                               min_d, max_d, pa, bnd, counters, bricks, nb);
         else
             hipLaunchKernelGGL(integrateFast<StoreF16>, grid, dim3(256), 0, st, (uint32_t*)vox, p, K, W, H, depth,
-                               min_d, max_d, pa, counters, bricks, nb);
+                               min_d, max_d, pa, bnd, counters, bricks, nb);
     }
     return hipGetLastError();
 }
