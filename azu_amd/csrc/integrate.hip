@@ -92,6 +92,8 @@ __device__ __forceinline__ void flushCounts(unsigned long long* counters, unsign
 }
 
 constexpr int kChunk = 32;  // x columns per tile
+constexpr int kSegChunks = 4;  // tiles per wave along a row (integrateExact)
+constexpr int kSegCols = kSegChunks * kChunk;
 constexpr int kSub = 16;    // columns whose depth gathers are issued together
 constexpr int kLdsStride = kChunk + 1;
 
@@ -302,19 +304,27 @@ __global__ __launch_bounds__(256) void integrateExact(typename S::Word* __restri
     // words is one 128-byte line); rows and columns outside the frustum box stay idle.
     // The 64-row alignment also measures faster than groups starting at the box (FP16
     // 290 against 347 us at 512^3 on the same frames).
+    // A row is cut into segments of kSegChunks tiles, one wave each, so that a frame has
+    // several waves per SIMD to hide its gathers behind; a wave first carries its rows'
+    // camera coordinates to its segment by the same serial x-steps the whole row takes.
     const int ny = f.y1 - f.y0 + 1;
     const int zbase = f.z0 & ~63;
     const int nzg = (f.z1 - zbase + 1 + 63) / 64;
+    const int xstart = f.x0 & ~(kChunk - 1);
+    const int nseg = (f.x1 - xstart + kSegCols) / kSegCols;
     const int task = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wv);  // wave-uniform
-    if (task >= ny * nzg) return;  // whole wave; the kernel uses no block barrier
-    const int y = f.y0 + task % ny;
-    const int zb = zbase + (task / ny) * 64;
+    if (task >= ny * nzg * nseg) return;  // whole wave; the kernel uses no block barrier
+    const int seg = task / (ny * nzg), rem = task % (ny * nzg);
+    const int y = f.y0 + rem % ny;
+    const int zb = zbase + (rem / ny) * 64;
+    const int xs = xstart + seg * kSegCols, xe = min(xs + kSegCols - 1, f.x1);
     const int za = max(zb, f.z0), ze = min(zb + 63, f.z1);
     const int z = zb + lane;
     const bool row_ok = z >= f.z0 && z <= f.z1;
     const float vs = p.vs, trunc = p.trunc;
     const F3 pp{p.origin[0] + (float)f.x0 * vs, p.origin[1] + (float)y * vs, p.origin[2] + (float)z * vs};
     F3 c = f3add(mulv(f.R_wc, pp), f.t_wc);
+    for (int x = f.x0; x < xs; ++x) c = f3add(c, f.step);
     float* L = lds[wv];
     const size_t res = (size_t)p.res, plane = res * res;
     const float fW = (float)W, fH = (float)H;
@@ -323,25 +333,23 @@ __global__ __launch_bounds__(256) void integrateExact(typename S::Word* __restri
     const float cbound = fabsf(f.t_wc.x) + fabsf(f.t_wc.y) + fabsf(f.t_wc.z) +
                          fabsf(p.origin[0]) + fabsf(p.origin[1]) + fabsf(p.origin[2]) + 3.0f * (float)p.res * vs;
 
-    // Classify every tile of the row at once, lane t taking tile t (at most 64 tiles: res
-    // <= 2048), so the depth-bounds reads of all tiles overlap; the classes come back as
-    // two wave masks.
-    const int xstart = f.x0 & ~(kChunk - 1);
+    // Classify every tile of the segment at once, lane t taking tile t, so the
+    // depth-bounds reads of all tiles overlap; the classes come back as two wave masks.
     unsigned long long cls_lo, cls_hi;
     {
         int my = kTileCull;
-        const int xt = xstart + lane * kChunk;
-        if (xt <= f.x1) {
+        const int xt = xs + lane * kChunk;
+        if (lane < kSegChunks && xt <= xe) {
             const int js = max(f.x0 - xt, 0), n = min(kChunk, f.x1 - xt + 1);
             my = tileClass(f, p, K, W, H, max_d, xt + js, xt + n - 1, y, za, ze, bnd, cbound);
         }
         cls_lo = __ballot((my & 1) != 0);
         cls_hi = __ballot((my & 2) != 0);
     }
-    for (int xc = xstart; xc <= f.x1; xc += kChunk) {
+    for (int xc = xs; xc <= xe; xc += kChunk) {
         // columns [js, n) of the tile are in the frustum box
         const int js = max(f.x0 - xc, 0), n = min(kChunk, f.x1 - xc + 1);
-        const int ti = (xc - xstart) / kChunk;
+        const int ti = (xc - xs) / kChunk;
         const int cls = (int)((cls_lo >> ti) & 1ull) | ((int)((cls_hi >> ti) & 1ull) << 1);
         if (cls != kTileExact) {
             // The row's camera coordinate still advances step by step.
@@ -746,9 +754,10 @@ hipError_t launchIntegrate(int format, bool exact, void* vox, const VolParams& p
     const int nb = brickCount(p.res);
     if (exact) {
         const DepthBounds bnd = boundsFor(bounds, depth, W, H, min_d, max_d, st);
-        // One wave per (y, 64-row z group) of the whole volume; waves outside the
-        // frustum box exit at once (the box is only known on the device).
-        const long long tasks = (long long)p.res * ((p.res + 63) / 64);
+        // One wave per (y, 64-row z group, row segment) of the whole volume; waves outside
+        // the frustum box exit at once (the box is only known on the device).
+        const long long tasks =
+            (long long)p.res * ((p.res + 63) / 64 + 1) * ((p.res + kSegCols - 1) / kSegCols + 1);
         const dim3 grid((unsigned)((tasks + 3) / 4)), block(256);
         // sdf = d - c.z with d >= min_d: when min_d >= 2^-60 a nonzero sdf cannot fall below
         // 2^-100 and the unguarded fixed-divisor form runs.
