@@ -13,7 +13,15 @@ on the device and is copied out asynchronously, so the timed region is GPU work 
 it. `roofline` is for the dominant kernel of the FP16 form (the one with the most
 time per frame), from HIP events on its launch stream; `cpu_baseline` times the CPU
 oracle (FP32 restatement of the reference CPU path with the reference's 20 B AoS
-voxel) on the first frames of the same sequence.
+voxel) on the first frames of the same sequence, and counts the voxels the raycast
+reads there (the raycast's algorithmic bytes).
+
+Beside the frame loop (skipped with --no-extras): the 1024^3 / 1280x960 form (20
+frames each of FP16 and FP32), the raycast-depth loss and its adjoint on the 512^3
+FP32 model (640x480 rays, step 0.5 voxel), and the raw-Kinect depth conditioning chain
+on one 640x480 frame. `accuracy_vs_fp64` is the FP16 form's error statement against
+the FP64 oracle as recorded in profiles/accuracy_512.json (the benchmark does not run
+the oracle outside its CPU-baseline leg).
 """
 import argparse
 import ctypes
@@ -41,6 +49,8 @@ def parse():
     ap.add_argument("--cpu-frames", type=int, default=8)
     ap.add_argument("--profile-frames", type=int, default=24)
     ap.add_argument("--only", choices=["both", "f16", "f32"], default="both")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the legs beside the frame loop (1024^3 form, adjoint, depth conditioning)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy kernel of MIB MiB 5 times (the known-size "
                          "reference for HBM byte counters) and exit")
@@ -65,7 +75,9 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_form(frames_dev, nframes, pose0, params, fmt, W, H, K, warmup, steps):
+def run_form(frames_dev, nframes, pose0, params, fmt, W, H, K, warmup, steps, keep=False):
+    """Times `steps` frames after `warmup`; with keep, also returns the open loop (its
+    volume holds the fused model, its tracker the last live frame)."""
     from azu_amd import native as N
     from azu_amd.pipeline import FrameLoop
 
@@ -81,8 +93,124 @@ def run_form(frames_dev, nframes, pose0, params, fmt, W, H, K, warmup, steps):
     N.synchronize()
     dt = time.perf_counter() - t0
     poses = loop.poses()
+    if keep:
+        return dt, poses, loop
     loop.close()
     return dt, poses
+
+
+def adjoint_leg(loop, pose, K, W, H, reps=5):
+    """The raycast-depth loss and its adjoint on the FP32 model the loop fused (512^3),
+    640 x 480 rays, step 0.5 voxel, observed depth = the last live frame: event-timed
+    forward (loss only) and forward + backward (pose gradient and the TSDF gradient
+    scattered into a res^3 float buffer)."""
+    from azu_amd import native as N
+    from azu_amd.fusion import pose_cm
+
+    d_obs = loop.tr.level(0)[0]
+    res = loop.vol.res
+    grad = N.DeviceBuffer(res ** 3 * 4)  # contents irrelevant to the timing (+= into it)
+    Kc = N.f32(K)
+    T = N.f32(pose_cm(pose))
+    out = (ctypes.c_double * 8)()
+
+    def call(backward):
+        N.check(N.lib.azu_raycast_depth_loss(loop.vol.h, N.fptr(T), N.fptr(Kc), W, H, d_obs, 0.5,
+                                             1 if backward else 0, grad.ptr if backward else None, None, out,
+                                             None))
+
+    call(False)
+    call(True)
+    N.profile_reset()
+    N.profile_enable(True)
+    for _ in range(reps):
+        call(False)
+    for _ in range(reps):
+        call(True)
+    prof = N.profile_read()
+    N.profile_enable(False)
+    grad.free()
+    fwd, bwd = prof.get("depth_loss_fwd"), prof.get("depth_loss_bwd")
+    return {"volume": "%d^3 fp32" % res, "rays": "%dx%d" % (W, H), "step_voxels": 0.5,
+            "forward_us": round(1000 * fwd[1] / fwd[0], 1) if fwd else None,
+            "forward_backward_us": round(1000 * bwd[1] / bwd[0], 1) if bwd else None,
+            "pixels": int(out[7]), "loss": float(out[0])}
+
+
+def conditioning_leg(frame_mm, reps=50):
+    """The raw-Kinect depth conditioning chain (median, hole fill, guided filter with a
+    guidance luma image, temporal EMA) on one 640 x 480 frame of raw codes made from the
+    sequence's first frame, per-stage event times averaged over reps frames."""
+    from azu_amd import native as N
+    from azu_amd.conditioner import SignalConditioner
+
+    H, W = frame_mm.shape
+    m = frame_mm.astype(np.float64) * 1e-3
+    with np.errstate(divide="ignore"):
+        code = (1.0 / np.where(m > 0, m, np.inf) - 3.3309495161) / -0.0030711016
+    raw = np.where(frame_mm > 0, np.clip(np.rint(code), 1, 2046), 0).astype(np.uint16)
+    yy, xx = np.mgrid[0:H, 0:W].astype(np.float32)
+    luma = np.clip(0.4 + 0.3 * xx / W + 0.1 * np.sin(yy / 23.0), 0.0, 1.0).astype(np.float32)
+    sc_ = SignalConditioner(W, H)
+    d_in, d_out, d_luma = N.device_array(raw), N.DeviceBuffer(raw.nbytes), N.device_array(luma)
+    for _ in range(3):
+        sc_.process_device(d_in.ptr, d_out.ptr, d_luma.ptr)
+    N.synchronize()
+    N.profile_reset()
+    N.profile_enable(True)
+    for _ in range(reps):
+        sc_.process_device(d_in.ptr, d_out.ptr, d_luma.ptr)
+    prof = N.profile_read()
+    N.profile_enable(False)
+    N.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        sc_.process_device(d_in.ptr, d_out.ptr, d_luma.ptr)
+    N.synchronize()
+    wall = (time.perf_counter() - t0) / reps
+    sc_.close()
+    stages = {k: round(1000 * v[1] / v[0], 2) for k, v in prof.items() if k.startswith("cond_")}
+    return {"frame": "%dx%d raw codes + luma" % (W, H), "us_per_frame": round(1e6 * wall, 1),
+            "stage_us": stages}
+
+
+def c5_leg(steps=20, warmup=2):
+    """The large-volume form: 1024^3 at 1 mm, 1280 x 960 frames (the 640 x 480 sequence
+    upsampled 2x bicubic), FP16 + UInt16 and FP32, `steps` timed frames each."""
+    from azu_amd import native as N
+    from azu_amd import scene as sc
+
+    w = sc.WORKLOADS["c5_1024"]
+    params = sc.volume_params("c5_1024")
+    frames_mm, poses_true, K = sc.sequence("c5_1024", frames=warmup + steps)
+    dev = [N.device_array(f) for f in frames_mm]
+    out = {"workload": "1024^3 TSDF at 1 mm, 1280x960 uint16 depth (2x bicubic), %d frames" % (warmup + steps)}
+    for name, fmt in (("f16", N.FMT_F16_U16), ("f32", N.FMT_F32)):
+        dt, poses = run_form([d.ptr for d in dev], warmup + steps, poses_true[0], params, fmt, w["W"], w["H"], K,
+                             warmup, steps)
+        terr = np.abs(poses[:, :3, 3] - poses_true[:warmup + steps, :3, 3]).max()
+        out[name] = {"fps": round(steps / dt, 2), "ms_per_step": round(1000 * dt / steps, 3),
+                     "max_track_err_m": float(terr)}
+        log("c5", name, out[name])
+    for d in dev:
+        d.free()
+    out["fp16_over_fp32"] = round(out["f16"]["fps"] / out["f32"]["fps"], 3)
+    return out
+
+
+def accuracy_statement():
+    """The FP16 + UInt16 form's error against the FP64 oracle, as measured by
+    tests/test_gpu_full_size.py::test_fp16_error_statement and recorded in
+    profiles/accuracy_512.json (the oracle is not run by the benchmark)."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "accuracy_512.json")) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    a, s = rec["fp16_vs_fp64_oracle_10_frames"], rec["fp16_vs_fp32_oracle_10_frames"]
+    return {"source": "profiles/accuracy_512.json (%s)" % rec["command"],
+            "fp16_vs_fp64": {k: a[k] for k in ("tsdf_rms", "depth_rms_m", "depth_p999_m", "hit_mismatch")},
+            "fp16_storage_only_vs_fp32": {k: s[k] for k in ("tsdf_rms", "tsdf_max", "depth_rms_m")}}
 
 
 def run_form_pcie(frames_mm, pose0, params, fmt, W, H, K, warmup, steps):
@@ -143,7 +271,7 @@ def profile_form(frames_dev, pose0, params, fmt, W, H, K, nframes):
     return prof, upd / nf, icp
 
 
-def rooflines(prof, nprof, n_upd, icp, s_v, W, H, res):
+def rooflines(prof, nprof, n_upd, icp, s_v, W, H, res, u_ray=None):
     """Achieved HBM GB/s of each hot kernel from its algorithmic bytes per frame and its
     event-timed duration per frame (equal to bytes per launch / average launch time)."""
     out = {}
@@ -155,6 +283,11 @@ def rooflines(prof, nprof, n_upd, icp, s_v, W, H, res):
             # the same time against the bytes of a read + write of the whole volume
             full = 2 * s_v * res ** 3 / (total_ms / launches * 1e-3) / 1e9
             extra = {"updated_voxels": int(n_upd), "full_sweep_equiv_frac": round(full / HBM_PEAK_GBS, 4)}
+        elif k.startswith("raycast") and u_ray:
+            # the distinct voxels the marches and normal stencils read, once each, plus the
+            # vertex and normal image written
+            b = s_v * u_ray + 24 * W * H
+            extra = {"touched_voxels": int(u_ray)}
         elif k == "icp_iter":
             # live vertex + normal per interior pixel, model vertex + normal per in-bounds
             # projection, and the per-block partials (34 doubles x 256 blocks, written and
@@ -192,8 +325,12 @@ def cpu_baseline(frames_mm, pose0, params, K, nframes):
     v = oracle.Volume(params["resolution"], params["voxel_size"], params["truncation"], params["max_weight"],
                       params["origin"], params["min_depth"], params["max_depth"], dtype=np.float32)
     t0 = time.perf_counter()
-    _, ms, _ = v.run_sequence(frames_mm[:nframes], K, pose0)
+    poses, ms, _ = v.run_sequence(frames_mm[:nframes], K, pose0)
     dt = time.perf_counter() - t0
+    H, W = frames_mm.shape[1:]
+    # Distinct voxels the last frame's raycast reads (the raycast's algorithmic bytes);
+    # counted on the oracle's model, which the FP32 kernels reproduce bit for bit.
+    u_ray = int(v.raycast_touched(poses[-1], K, W, H))
     v.close()
     # frame 0 has no ICP; the rate is over frames 1.. (median frame time)
     med = float(np.median(ms[1:, 4])) if nframes > 1 else float(ms[0, 4])
@@ -201,7 +338,7 @@ def cpu_baseline(frames_mm, pose0, params, K, nframes):
             "sample": "frames 1..%d of the same %d^3 sequence (%.1f s wall), median frame time %.1f ms "
                       "(pyramid %.1f, icp %.1f, integrate %.1f, raycast %.1f ms)" % (
                           nframes - 1, params["resolution"], dt, med, np.median(ms[1:, 0]), np.median(ms[1:, 1]),
-                          np.median(ms[1:, 2]), np.median(ms[1:, 3]))}
+                          np.median(ms[1:, 2]), np.median(ms[1:, 3])), "_u_ray": u_ray}
 
 
 def main():
@@ -231,9 +368,16 @@ def main():
     log("device", dev_name)
 
     results = {}
+    adjoint = None
     forms = {"f16": N.FMT_F16_U16, "f32": N.FMT_F32}
     for name in (["f16", "f32"] if args.only == "both" else [args.only]):
-        dt, poses = run_form(frames_dev, nframes, pose0, params, forms[name], W, H, K, args.warmup, args.steps)
+        keep = name == "f32" and not args.no_extras
+        r = run_form(frames_dev, nframes, pose0, params, forms[name], W, H, K, args.warmup, args.steps, keep=keep)
+        dt, poses = r[0], r[1]
+        if keep:
+            adjoint = adjoint_leg(r[2], poses[-1], K, W, H)
+            r[2].close()
+            log("adjoint", adjoint)
         terr = np.abs(poses[:, :3, 3] - poses_true[:nframes, :3, 3]).max(axis=1)
         results[name] = {"seconds": dt, "fps": args.steps / dt, "ms_per_step": 1000.0 * dt / args.steps,
                          "max_track_err_m": float(terr.max())}
@@ -244,14 +388,24 @@ def main():
     pcie = run_form_pcie(frames_mm, pose0, params, forms[head], W, H, K, args.warmup, args.steps)
     log("%s with per-frame upload from pinned host memory: %.1f frames/s" % (head, pcie))
 
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(frames_mm, pose0, params, K, args.cpu_frames)
+        log("cpu baseline", cpu)
+    u_ray = cpu.pop("_u_ray") if cpu else None
+
     # Per-kernel timing (fresh run, events on the launch stream) for the roofline.
-    prof, n_upd, icp = profile_form(frames_dev, pose0, params, forms[head], W, H, K,
-                                    min(args.profile_frames, len(frames_dev)))
-    nprof = max(min(args.profile_frames, len(frames_dev)) - 1, 1)
+    nprof_frames = max(min(args.profile_frames, len(frames_dev)), 2)
+    prof, n_upd, icp = profile_form(frames_dev, pose0, params, forms[head], W, H, K, nprof_frames)
+    nprof = nprof_frames - 1
     per_frame = {k: v[1] / nprof for k, v in prof.items()}
-    dom = max(per_frame, key=per_frame.get)
+    # The dominant kernel among the one-launch-per-frame kernels: the ICP's 20 launches a
+    # frame each carry their own event pair, whose recording gaps the per-frame sum of
+    # icp_iter includes (its rocprof kernel time per frame is below integrate's).
+    single = {k: v for k, v in per_frame.items() if k.startswith(("integrate", "raycast"))}
+    dom = max(single or per_frame, key=(single or per_frame).get)
     s_v = 4 if head == "f16" else 8
-    roofs = rooflines(prof, nprof, n_upd, icp, s_v, W, H, w["res"])
+    roofs = rooflines(prof, nprof, n_upd, icp, s_v, W, H, w["res"], u_ray)
     roof = roofs.get(dom) or {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
                               "traffic": None, "kernel": dom,
                               "avg_launch_us": round(1000 * prof[dom][1] / prof[dom][0], 2)}
@@ -263,10 +417,13 @@ def main():
                 "us_per_frame": round(1000 * v[1] / nprof, 2)} for k, v in prof.items()}
     log("kernels:", json.dumps(kern))
 
-    cpu = None
-    if not args.no_cpu_baseline:
-        cpu = cpu_baseline(frames_mm, pose0, params, K, args.cpu_frames)
-        log("cpu baseline", cpu)
+    extras = {}
+    if not args.no_extras:
+        extras["depth_conditioning"] = conditioning_leg(frames_mm[0])
+        log("conditioning", extras["depth_conditioning"])
+        for d in dev:
+            d.free()
+        extras["c5_1024"] = c5_leg()
 
     r = results[head]
     line = {
@@ -299,8 +456,13 @@ def main():
         "max_track_err_m": r["max_track_err_m"],
         "pcie_inclusive_fps": round(pcie, 2),
         "roofline": roof,
-        "roofline_by_kernel": {k: {kk: v[kk] for kk in ("achieved", "frac", "avg_launch_us", "full_sweep_equiv_frac")
+        "roofline_by_kernel": {k: {kk: v[kk] for kk in ("achieved", "frac", "avg_launch_us", "full_sweep_equiv_frac",
+                                                        "algorithmic_bytes_per_launch")
                                    if kk in v} for k, v in roofs.items()},
+        "accuracy_vs_fp64": accuracy_statement(),
+        "c5_1024": extras.get("c5_1024"),
+        "adjoint_c4": adjoint,
+        "depth_conditioning": extras.get("depth_conditioning"),
         "kernels_us_per_frame": {k: v["us_per_frame"] for k, v in kern.items()},
         "cpu_baseline": cpu,
     }
