@@ -7,11 +7,13 @@
 //    camera coordinate by an incremental x-step (c += R_wc.col(0) * vs,
 //    TSDFVolume.cpp:286-287), so voxel x's coordinate is the result of (x - x0)
 //    sequential float adds. To reproduce it bit for bit while keeping HBM access
-//    coalesced, every wave owns 64 rows (consecutive z at one y): in phase 1 lane r
-//    walks row r through a 32-column chunk, runs the projection, depth gather and the
-//    gates, and leaves tsdf_new (or NaN = skip) in LDS; in phase 2 the wave reads the
-//    chunk transposed, lane = column, so each read-modify-write covers 32 consecutive
-//    voxels of two rows. Division and square root are IEEE-exact, nothing contracts.
+//    coalesced, every wave owns 32 rows (consecutive z at one y): in phase 1 lanes r and
+//    r + 32 walk row r through the two halves of a 32-column chunk, run the projection,
+//    depth gather and the gates, and leave tsdf_new (or NaN = skip) in LDS; in phase 2
+//    the wave reads the chunk transposed, lane = column, so each read-modify-write
+//    covers 32 consecutive voxels of two rows. 32-row tiles keep a block's LDS at 17 KB,
+//    so that up to 8 waves a SIMD fit (64-row tiles allowed 4 and measured 390 against
+//    260 us when held to 2). Division and square root are IEEE-exact, nothing contracts.
 //
 //  * integrateFast<Store>: the FP16 + UInt16 storage form. The same tiles and tile
 //    classes, but lane = x column: the camera coordinate is an affine function of
@@ -93,8 +95,11 @@ __device__ __forceinline__ void flushCounts(unsigned long long* counters, unsign
 
 constexpr int kChunk = 32;  // x columns per tile
 constexpr int kSegChunks = 4;  // tiles per wave along a row (integrateExact)
+constexpr int kTileRows = 32;  // z rows per tile (integrateExact)
+constexpr int kFoldBatch = 8;  // row pairs whose words one phase-2 step loads together
 constexpr int kSegCols = kSegChunks * kChunk;
-constexpr int kSub = 16;    // columns whose depth gathers are issued together
+constexpr int kSub = 16;    // columns of a tile half (one phase-1 lane)
+constexpr int kGather = 8;  // columns whose depth gathers are issued together
 constexpr int kLdsStride = kChunk + 1;
 
 // Depth bounds of the frame: per cell of 2^l x 2^l pixels (l = 1..kBoundLevels), the
@@ -290,36 +295,38 @@ __device__ __forceinline__ void markTile(unsigned char* __restrict__ bricks, int
 }
 
 template <class S, int FD, bool RC>
-__global__ __launch_bounds__(256) void integrateExact(typename S::Word* __restrict__ vox, VolParams p, Intr K,
+__global__ __launch_bounds__(256, 8) void integrateExact(typename S::Word* __restrict__ vox, VolParams p, Intr K,
                                                       int W, int H, const float* __restrict__ depth, float min_d,
                                                       float max_d, PoseArg pa, FixedDiv dtr, DepthBounds bnd,
                                                       unsigned long long* __restrict__ counters,
                                                       unsigned char* __restrict__ bricks, int nb) {
-    __shared__ float lds[4][64 * kLdsStride];
+    __shared__ float lds[4][kTileRows * kLdsStride];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const IntFrame f = integrateSetup(p, pa, K, W, H, max_d);
     if (!f.ok) return;
-    // Tiles start on multiples of 64 rows in z and of kChunk columns in x (rows and
-    // columns of a tile then fall into bricks by their index alone, and a tile row of FP16
-    // words is one 128-byte line); rows and columns outside the frustum box stay idle.
-    // The 64-row alignment also measures faster than groups starting at the box (FP16
-    // 290 against 347 us at 512^3 on the same frames).
+    // Tiles start on multiples of kTileRows rows in z and of kChunk columns in x (rows
+    // and columns of a tile then fall into bricks by their index alone, and a tile row of
+    // FP16 words is one 128-byte line); rows and columns outside the frustum box stay
+    // idle. The aligned groups also measure faster than groups starting at the box (FP16
+    // 290 against 347 us at 512^3 on the same frames, with 64-row tiles).
     // A row is cut into segments of kSegChunks tiles, one wave each, so that a frame has
     // several waves per SIMD to hide its gathers behind; a wave first carries its rows'
     // camera coordinates to its segment by the same serial x-steps the whole row takes.
     const int ny = f.y1 - f.y0 + 1;
-    const int zbase = f.z0 & ~63;
-    const int nzg = (f.z1 - zbase + 1 + 63) / 64;
+    const int zbase = f.z0 & ~(kTileRows - 1);
+    const int nzg = (f.z1 - zbase + kTileRows) / kTileRows;
     const int xstart = f.x0 & ~(kChunk - 1);
     const int nseg = (f.x1 - xstart + kSegCols) / kSegCols;
     const int task = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wv);  // wave-uniform
     if (task >= ny * nzg * nseg) return;  // whole wave; the kernel uses no block barrier
     const int seg = task / (ny * nzg), rem = task % (ny * nzg);
     const int y = f.y0 + rem % ny;
-    const int zb = zbase + (rem / ny) * 64;
+    const int zb = zbase + (rem / ny) * kTileRows;
     const int xs = xstart + seg * kSegCols, xe = min(xs + kSegCols - 1, f.x1);
-    const int za = max(zb, f.z0), ze = min(zb + 63, f.z1);
-    const int z = zb + lane;
+    const int za = max(zb, f.z0), ze = min(zb + kTileRows - 1, f.z1);
+    // Phase-1 lanes: row r of the tile, column half h (columns 16 h .. 16 h + 15).
+    const int r = lane & (kTileRows - 1), h = lane >> 5;
+    const int z = zb + r;
     const bool row_ok = z >= f.z0 && z <= f.z1;
     const float vs = p.vs, trunc = p.trunc;
     const F3 pp{p.origin[0] + (float)f.x0 * vs, p.origin[1] + (float)y * vs, p.origin[2] + (float)z * vs};
@@ -356,38 +363,48 @@ __global__ __launch_bounds__(256) void integrateExact(typename S::Word* __restri
             for (int j = js; j < n; ++j) c = f3add(c, f.step);
             if (cls == kTileCull) continue;
         }
-        // Phase 1 (exact tiles): lane = row, serial over the tile's columns. Per
-        // sub-chunk, the projections first, then all depth gathers at once, then the gates.
+        // Phase 1 (exact tiles): lane = (row, column half), serial over the half's 16
+        // columns: the projections first, then all depth gathers at once, then the gates.
+        // The second-half lane first carries its row across the first half's columns (the
+        // same adds the first-half lane makes while it projects them), and afterwards both
+        // lanes of the row continue from its coordinate, which has taken every column.
+        if (cls == kTileExact) {
+            for (int j = js; j < min(n, kSub); ++j)
+                if (h) c = f3add(c, f.step);
 #pragma unroll 1
-        for (int j0 = 0; j0 < (cls == kTileExact ? kChunk : 0); j0 += kSub) {
-            float cz[kSub], dj[kSub];
-            int pix[kSub];
+            for (int j0 = h * kSub; j0 < h * kSub + kSub; j0 += kGather) {
+                float cz[kGather], dj[kGather];
+                int pix[kGather];
 #pragma unroll
-            for (int jj = 0; jj < kSub; ++jj) {
-                pix[jj] = -1;
-                cz[jj] = c.z;
-                if (j0 + jj >= js && j0 + jj < n) {
-                    if (row_ok && c.z > 0.0f) {
-                        const float inv_z = recipBy<RC>(c.z);
-                        const float uf = floorf(K.fx * c.x * inv_z + K.cx + 0.5f);
-                        const float vf = floorf(K.fy * c.y * inv_z + K.cy + 0.5f);
-                        if (uf >= 0.0f && uf < fW && vf >= 0.0f && vf < fH) pix[jj] = (int)vf * W + (int)uf;
+                for (int jj = 0; jj < kGather; ++jj) {
+                    pix[jj] = -1;
+                    cz[jj] = c.z;
+                    if (j0 + jj >= js && j0 + jj < n) {
+                        if (row_ok && c.z > 0.0f) {
+                            const float inv_z = recipBy<RC>(c.z);
+                            const float uf = floorf(K.fx * c.x * inv_z + K.cx + 0.5f);
+                            const float vf = floorf(K.fy * c.y * inv_z + K.cy + 0.5f);
+                            if (uf >= 0.0f && uf < fW && vf >= 0.0f && vf < fH) pix[jj] = (int)vf * W + (int)uf;
+                        }
+                        c = f3add(c, f.step);
                     }
-                    c = f3add(c, f.step);
+                }
+#pragma unroll
+                for (int jj = 0; jj < kGather; ++jj) dj[jj] = depth[pix[jj] < 0 ? 0 : pix[jj]];
+#pragma unroll
+                for (int jj = 0; jj < kGather; ++jj) {
+                    float tn = __builtin_nanf("");
+                    const float d = dj[jj];
+                    if (pix[jj] >= 0 && d >= min_d && d <= max_d) {
+                        const float sdf = d - cz[jj];
+                        if (!(sdf < -trunc)) tn = smin(1.0f, divBy<FD>(dtr, sdf));
+                    }
+                    L[r * kLdsStride + j0 + jj] = tn;
                 }
             }
-#pragma unroll
-            for (int jj = 0; jj < kSub; ++jj) dj[jj] = depth[pix[jj] < 0 ? 0 : pix[jj]];
-#pragma unroll
-            for (int jj = 0; jj < kSub; ++jj) {
-                float tn = __builtin_nanf("");
-                const float d = dj[jj];
-                if (pix[jj] >= 0 && d >= min_d && d <= max_d) {
-                    const float sdf = d - cz[jj];
-                    if (!(sdf < -trunc)) tn = smin(1.0f, divBy<FD>(dtr, sdf));
-                }
-                L[lane * kLdsStride + j0 + jj] = tn;
-            }
+            c.x = __shfl(c.x, r + 32);
+            c.y = __shfl(c.y, r + 32);
+            c.z = __shfl(c.z, r + 32);
         }
         waveSync();
         // Phase 2: lane = column, two rows per instruction; per half of the rows, every
@@ -396,8 +413,8 @@ __global__ __launch_bounds__(256) void integrateExact(typename S::Word* __restri
         const bool col_ok = j >= js && j < n;
         // The tile's words through a buffer resource: base = the tile's first voxel
         // (wave-uniform), lane offset (g plane + j) words, row offset 2 r plane words
-        // (uniform). The range is capped at 2^31 bytes (64 planes of at most 2048^2
-        // words stay below it), so the lane offset 2^31 reads 0 with no memory access.
+        // (uniform). The range is capped at 2^31 bytes (a tile's 32 planes of at most
+        // 2048^2 words stay below it), so the lane offset 2^31 reads 0 with no memory access.
         const size_t tile0 = (size_t)zb * plane + (size_t)y * res + (size_t)xc;
         const size_t left = (res * plane - tile0) * sizeof(typename S::Word);
         const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
@@ -405,32 +422,32 @@ __global__ __launch_bounds__(256) void integrateExact(typename S::Word* __restri
         const unsigned lane_off = (unsigned)(((size_t)g * plane + (size_t)j) * sizeof(typename S::Word));
         const unsigned row_step = (unsigned)(2 * plane * sizeof(typename S::Word));
         // rows r with zb + g + 2 r in [z0, z1] (none for a column outside the box)
-        const int r_lo = col_ok ? max(0, (f.z0 - zb - g + 1) >> 1) : 32;
-        const int r_hi = min(31, (f.z1 - zb - g) >> 1);
+        const int r_lo = col_ok ? max(0, (f.z0 - zb - g + 1) >> 1) : kTileRows / 2;
+        const int r_hi = min(kTileRows / 2 - 1, (f.z1 - zb - g) >> 1);
+        // Brick map: with the tile aligned, row pair kk of either lane half lies in z-brick
+        // zb / 16 for kk < 8 and the next one for kk >= 8; near_h are the lanes that left a
+        // non-far voxel there, shared_h the lower-half lanes whose voxel at z = 16 q (pair
+        // kk = 8 h) was non-far (that layer is also the brick below's).
+        unsigned long long near0 = 0, near1 = 0, shared0 = 0, shared1 = 0;
 #pragma unroll 1
-        for (int k0 = 0; k0 < 32; k0 += 16) {
-            float tn[16];
-            typename S::Word wd[16];
+        for (int k0 = 0; k0 < kTileRows / 2; k0 += kFoldBatch) {
+            float tn[kFoldBatch];
+            typename S::Word wd[kFoldBatch];
 #pragma unroll
-            for (int k = 0; k < 16; ++k) {
-                const int r = g + 2 * (k0 + k);
-                const float t = cls == kTileFree ? 1.0f : L[r * kLdsStride + j];
+            for (int k = 0; k < kFoldBatch; ++k) {
+                const int row = g + 2 * (k0 + k);
+                const float t = cls == kTileFree ? 1.0f : L[row * kLdsStride + j];
                 tn[k] = (k0 + k >= r_lo && k0 + k <= r_hi) ? t : __builtin_nanf("");
             }
             // Lanes with nothing to fold read out of range (no access), so the loads
             // carry no branch and issue back to back.
 #pragma unroll
-            for (int k = 0; k < 16; ++k)
+            for (int k = 0; k < kFoldBatch; ++k)
                 wd[k] = BufWord<S>::load(rs, __builtin_isnan(tn[k]) ? 0x80000000u : lane_off,
                                          (unsigned)(k0 + k) * row_step);
-            // Brick map: with the tile aligned, rows k0 + k of either lane half lie in
-            // z-brick zb / 16 + k0 / 8 for k < 8 and the next one for k >= 8; near_h are the
-            // lanes that left a non-far voxel there, shared_h the lower-half lanes whose
-            // voxel at z = 16 q (row k = 8 h) was non-far (that layer is also the brick
-            // below's).
-            unsigned long long near0 = 0, near1 = 0, shared0 = 0, shared1 = 0;
 #pragma unroll
-            for (int k = 0; k < 16; ++k) {
+            for (int k = 0; k < kFoldBatch; ++k) {
+                const int kk = k0 + k;
                 const bool upd = !__builtin_isnan(tn[k]);
                 bool newly = false, nf = false;
                 if (upd) {
@@ -439,22 +456,22 @@ __global__ __launch_bounds__(256) void integrateExact(typename S::Word* __restri
                     newly = (w_old == 0.0f);
                     const float t_new = (t_old * w_old + tn[k]) / (w_old + 1.0f);
                     BufWord<S>::store(S::pack(t_new, smin(w_old + 1.0f, p.max_w)), rs, lane_off,
-                                      (unsigned)(k0 + k) * row_step);
+                                      (unsigned)kk * row_step);
                     nf = !S::farPacked(t_new);  // the new weight is >= 1
                 }
                 n_upd += upd ? 1u : 0u;
                 n_new += newly ? 1u : 0u;
                 if (bricks) {
                     const unsigned long long m = __ballot(nf);
-                    if (k < 8) near0 |= m;
+                    if (kk < 8) near0 |= m;
                     else near1 |= m;
-                    if (k == 0) shared0 = __ballot(nf && g == 0);
-                    if (k == 8) shared1 = __ballot(nf && g == 0);
+                    if (kk == 0) shared0 = __ballot(nf && g == 0);
+                    if (kk == 8) shared1 = __ballot(nf && g == 0);
                 }
             }
-            if (bricks && lane == 0 && (near0 | near1) != 0)
-                markTile(bricks, nb, xc / kBrick, y, zb / kBrick + k0 / 8, near0, near1, shared0, shared1);
         }
+        if (bricks && lane == 0 && (near0 | near1) != 0)
+            markTile(bricks, nb, xc / kBrick, y, zb / kBrick, near0, near1, shared0, shared1);
         waveSync();
     }
     if (counters) {
@@ -754,10 +771,10 @@ hipError_t launchIntegrate(int format, bool exact, void* vox, const VolParams& p
     const int nb = brickCount(p.res);
     if (exact) {
         const DepthBounds bnd = boundsFor(bounds, depth, W, H, min_d, max_d, st);
-        // One wave per (y, 64-row z group, row segment) of the whole volume; waves outside
-        // the frustum box exit at once (the box is only known on the device).
+        // One wave per (y, kTileRows-row z group, row segment) of the whole volume; waves
+        // outside the frustum box exit at once (the box is only known on the device).
         const long long tasks =
-            (long long)p.res * ((p.res + 63) / 64 + 1) * ((p.res + kSegCols - 1) / kSegCols + 1);
+            (long long)p.res * ((p.res + kTileRows - 1) / kTileRows + 1) * ((p.res + kSegCols - 1) / kSegCols + 1);
         const dim3 grid((unsigned)((tasks + 3) / 4)), block(256);
         // sdf = d - c.z with d >= min_d: when min_d >= 2^-60 a nonzero sdf cannot fall below
         // 2^-100 and the unguarded fixed-divisor form runs.
