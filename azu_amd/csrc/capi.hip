@@ -22,8 +22,9 @@ namespace azu {
 hipError_t launchIntegrate(int format, bool exact, void* vox, const VolParams& p, Intr K, int W, int H,
                            const float* depth, float min_d, float max_d, const PoseArg& pa, const FixedDiv& dtr,
                            bool recip, void* bounds, unsigned long long* counters, unsigned char* bricks,
-                           hipStream_t st);
+                           unsigned char* classes, hipStream_t st);
 size_t depthBoundsBytes(int W, int H);
+size_t tileClassBytes(int res);
 hipError_t launchFillEmpty(int format, void* vox, size_t n, hipStream_t st);
 hipError_t launchBrickMap(int format, const void* vox, int res, unsigned char* bricks, bool scan, hipStream_t st);
 hipError_t launchRaycast(int format, bool exact, const void* vox, const VolParams& p, const FixedDiv& dvs,
@@ -250,6 +251,8 @@ struct azu_volume {
     bool recip = false;                                // verified Newton-Raphson 1 / x
     void* bounds = nullptr;                            // depth bounds of the last frame size
     size_t bounds_bytes = 0;
+    unsigned char* classes = nullptr;                  // exact integration's tile-class map
+    size_t classes_bytes = 0;
     // Brick map for the raycast's empty-space skipping, kept by reset / upload /
     // integrate. Once the raw storage has been handed out, other writers may exist and
     // the map is rebuilt from the voxels before every raycast.
@@ -429,6 +432,7 @@ int azu_vol_destroy(azu_volume* v) {
     if (v->adj_partials) (void)hipFree(v->adj_partials);
     if (v->adj_out) (void)hipFree(v->adj_out);
     if (v->bounds) (void)hipFree(v->bounds);
+    if (v->classes) (void)hipFree(v->classes);
     delete v;
     return AZU_OK;
 }
@@ -696,11 +700,21 @@ static int integrateImpl(azu_volume* v, const float* d_depth, int W, int H, cons
             if (hipMalloc(&v->bounds, need) != hipSuccess) return fail(AZU_ERR_ALLOC, "depth bounds allocation failed");
             v->bounds_bytes = need;
         }
+        const size_t cneed = tileClassBytes(v->params.resolution);
+        if (exact && cneed > v->classes_bytes) {
+            AZU_HIP_TRY(hipDeviceSynchronize(), "synchronise before tile-class map reallocation");
+            if (v->classes) (void)hipFree(v->classes);
+            v->classes = nullptr;
+            v->classes_bytes = 0;
+            if (hipMalloc((void**)&v->classes, cneed) != hipSuccess)
+                return fail(AZU_ERR_ALLOC, "tile-class map allocation failed");
+            v->classes_bytes = cneed;
+        }
     }
     AZU_HIP_TRY(launchIntegrate(v->format, exact, v->vox, toVol(v->params), Intr{K[0], K[1], K[2], K[3]}, W, H,
                                 d_depth, min_d, max_d, pa, v->dtr, v->recip,
-                                bounds_scratch ? bounds_scratch : v->bounds, v->counters,
-                                v->bricks, (hipStream_t)s),
+                                bounds_scratch ? bounds_scratch : v->bounds, v->counters, v->bricks,
+                                bounds_scratch ? nullptr : v->classes, (hipStream_t)s),
                 "integrate launch");
     v->frames += 1;
     return AZU_OK;

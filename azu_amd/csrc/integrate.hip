@@ -249,6 +249,41 @@ __device__ __forceinline__ int tileClass(const IntFrame& f, const VolParams& p, 
 
 static_assert(kChunk == 2 * kBrick, "a tile spans two bricks in x");
 
+// Class map of integrateExact's tiles: one byte per 32-column x kTileRows-row tile at
+// one y, indexed by the tile's absolute position; a frame writes the entries of its
+// frustum box, and only those are read.
+__host__ __device__ inline int tileGrid(int res) { return (res + kChunk - 1) / kChunk; }
+__host__ __device__ inline size_t tileIndex(int res, int xt, int y, int zb) {
+    const size_t g = (size_t)tileGrid(res);
+    return ((size_t)y * g + (size_t)(zb / kTileRows)) * g + (size_t)(xt / kChunk);
+}
+size_t tileClassBytes(int res) { return (size_t)res * tileGrid(res) * tileGrid(res); }
+
+// One lane per tile of the frame's frustum box: tileClass with the arguments
+// integrateExact would pass, so the map holds the classes it would compute itself.
+__global__ __launch_bounds__(256) void tileClassKernel(VolParams p, Intr K, int W, int H, float max_d, PoseArg pa,
+                                                       DepthBounds bnd, unsigned char* __restrict__ classes) {
+    const IntFrame f = integrateSetup(p, pa, K, W, H, max_d);
+    if (!f.ok) return;
+    const int ny = f.y1 - f.y0 + 1;
+    const int zbase = f.z0 & ~(kTileRows - 1);
+    const int nzg = (f.z1 - zbase + kTileRows) / kTileRows;
+    const int xstart = f.x0 & ~(kChunk - 1);
+    const int ntx = (f.x1 - xstart + kChunk) / kChunk;
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (long long)ny * nzg * ntx) return;
+    const int xt = xstart + (int)(i % ntx) * kChunk;
+    const long long rest = i / ntx;
+    const int y = f.y0 + (int)(rest % ny);
+    const int zb = zbase + (int)(rest / ny) * kTileRows;
+    const int za = max(zb, f.z0), ze = min(zb + kTileRows - 1, f.z1);
+    const int js = max(f.x0 - xt, 0), n = min(kChunk, f.x1 - xt + 1);
+    const float cbound = fabsf(f.t_wc.x) + fabsf(f.t_wc.y) + fabsf(f.t_wc.z) + fabsf(p.origin[0]) +
+                         fabsf(p.origin[1]) + fabsf(p.origin[2]) + 3.0f * (float)p.res * p.vs;
+    classes[tileIndex(p.res, xt, y, zb)] =
+        (unsigned char)tileClass(f, p, K, W, H, max_d, xt + js, xt + n - 1, y, za, ze, bnd, cbound);
+}
+
 // One voxel word through a raw buffer resource (byte offsets: lane + uniform).
 template <class S> struct BufWord;
 template <> struct BufWord<StoreF16> {
@@ -299,7 +334,8 @@ __global__ __launch_bounds__(256, 8) void integrateExact(typename S::Word* __res
                                                       int W, int H, const float* __restrict__ depth, float min_d,
                                                       float max_d, PoseArg pa, FixedDiv dtr, DepthBounds bnd,
                                                       unsigned long long* __restrict__ counters,
-                                                      unsigned char* __restrict__ bricks, int nb) {
+                                                      unsigned char* __restrict__ bricks, int nb,
+                                                      const unsigned char* __restrict__ classes) {
     __shared__ float lds[4][kTileRows * kLdsStride];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const IntFrame f = integrateSetup(p, pa, K, W, H, max_d);
@@ -340,15 +376,20 @@ __global__ __launch_bounds__(256, 8) void integrateExact(typename S::Word* __res
     const float cbound = fabsf(f.t_wc.x) + fabsf(f.t_wc.y) + fabsf(f.t_wc.z) +
                          fabsf(p.origin[0]) + fabsf(p.origin[1]) + fabsf(p.origin[2]) + 3.0f * (float)p.res * vs;
 
-    // Classify every tile of the segment at once, lane t taking tile t, so the
-    // depth-bounds reads of all tiles overlap; the classes come back as two wave masks.
+    // The classes of the segment's tiles, lane t taking tile t: read from the frame's
+    // class map (tileClassKernel), or, without one, classified here, all tiles at once so
+    // that their depth-bounds reads overlap. They come back as two wave masks.
     unsigned long long cls_lo, cls_hi;
     {
         int my = kTileCull;
         const int xt = xs + lane * kChunk;
         if (lane < kSegChunks && xt <= xe) {
-            const int js = max(f.x0 - xt, 0), n = min(kChunk, f.x1 - xt + 1);
-            my = tileClass(f, p, K, W, H, max_d, xt + js, xt + n - 1, y, za, ze, bnd, cbound);
+            if (classes) {
+                my = classes[tileIndex(p.res, xt, y, zb)];
+            } else {
+                const int js = max(f.x0 - xt, 0), n = min(kChunk, f.x1 - xt + 1);
+                my = tileClass(f, p, K, W, H, max_d, xt + js, xt + n - 1, y, za, ze, bnd, cbound);
+            }
         }
         cls_lo = __ballot((my & 1) != 0);
         cls_hi = __ballot((my & 2) != 0);
@@ -763,11 +804,12 @@ static DepthBounds boundsFor(void* bounds, const float* depth, int W, int H, flo
     return bnd;
 }
 
-// bricks: the brick map to keep (brickCount(res)^3 bytes), or null.
+// bricks: the brick map to keep (brickCount(res)^3 bytes), or null. classes: the exact
+// kernel's tile-class map (tileClassBytes(res)), or null to classify inside the kernel.
 hipError_t launchIntegrate(int format, bool exact, void* vox, const VolParams& p, Intr K, int W, int H,
                            const float* depth, float min_d, float max_d, const PoseArg& pa, const FixedDiv& dtr,
                            bool recip, void* bounds, unsigned long long* counters, unsigned char* bricks,
-                           hipStream_t st) {
+                           unsigned char* classes, hipStream_t st) {
     const int nb = brickCount(p.res);
     if (exact) {
         const DepthBounds bnd = boundsFor(bounds, depth, W, H, min_d, max_d, st);
@@ -779,9 +821,16 @@ hipError_t launchIntegrate(int format, bool exact, void* vox, const VolParams& p
         // sdf = d - c.z with d >= min_d: when min_d >= 2^-60 a nonzero sdf cannot fall below
         // 2^-100 and the unguarded fixed-divisor form runs.
         const bool fd = dtr.fast != 0, rc = recip, far = min_d >= 0x1p-60f;
+        const unsigned char* cmap = nullptr;
+        if (classes && bnd.lvl) {
+            const long long tiles = (long long)p.res * tileGrid(p.res) * tileGrid(p.res);
+            hipLaunchKernelGGL(tileClassKernel, dim3((unsigned)((tiles + 255) / 256)), dim3(256), 0, st, p, K, W, H,
+                               max_d, pa, bnd, classes);
+            cmap = classes;
+        }
 #define AZU_INT_EXACT(S, W, FD, RC)                                                                        \
     hipLaunchKernelGGL((integrateExact<S, FD, RC>), grid, block, 0, st, (W*)vox, p, K, W_, H, depth, min_d, \
-                       max_d, pa, dtr, bnd, counters, bricks, nb)
+                       max_d, pa, dtr, bnd, counters, bricks, nb, cmap)
         const int W_ = W;
         if (format == kFmtF32) {
             if (fd && far && rc) AZU_INT_EXACT(StoreF32, float2, 2, true);
