@@ -75,6 +75,38 @@ __device__ __forceinline__ TriSample sampleGrad(const AdjVol<S>& V, F3 p) {
     return s;
 }
 
+// The value alone (the march): the same cell, gate and interpolation as sampleGrad, its
+// eight corners read as four x-adjacent pairs.
+template <class S>
+__device__ __forceinline__ bool sampleValue(const AdjVol<S>& V, F3 p, float& f) {
+    const float gx = (p.x - V.ox) * V.inv_vs, gy = (p.y - V.oy) * V.inv_vs, gz = (p.z - V.oz) * V.inv_vs;
+    const float fx0 = floorf(gx), fy0 = floorf(gy), fz0 = floorf(gz);
+    const float top = (float)(V.res - 1);
+    if (!(fx0 >= 0.0f && fy0 >= 0.0f && fz0 >= 0.0f && fx0 < top && fy0 < top && fz0 < top)) return false;
+    const size_t sy = (size_t)V.res, sz = sy * sy;
+    const size_t b = (size_t)fz0 * sz + (size_t)fy0 * sy + (size_t)fx0;
+    typename S::Word w[8];
+    __builtin_memcpy(&w[0], V.vox + b, 2 * sizeof(typename S::Word));
+    __builtin_memcpy(&w[2], V.vox + b + sy, 2 * sizeof(typename S::Word));
+    __builtin_memcpy(&w[4], V.vox + b + sz, 2 * sizeof(typename S::Word));
+    __builtin_memcpy(&w[6], V.vox + b + sz + sy, 2 * sizeof(typename S::Word));
+    bool ok = true;
+    float c[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        ok = ok && S::weight(w[i]) > 0.0f;
+        c[i] = S::tsdf(w[i]);
+    }
+    if (!ok) return false;
+    const float tx = gx - fx0, ty = gy - fy0, tz = gz - fz0;
+    const float ux = 1.0f - tx, uy = 1.0f - ty, uz = 1.0f - tz;
+    const float v00 = c[0] * ux + c[1] * tx, v10 = c[2] * ux + c[3] * tx;
+    const float v01 = c[4] * ux + c[5] * tx, v11 = c[6] * ux + c[7] * tx;
+    const float v0 = v00 * uy + v10 * ty, v1 = v01 * uy + v11 * ty;
+    f = v0 * uz + v1 * tz;
+    return true;
+}
+
 __device__ __forceinline__ void scatterTrilinear(float* grad, size_t b, size_t sy, size_t sz, float tx, float ty,
                                                  float tz, float g) {
     const float ux = 1.0f - tx, uy = 1.0f - ty, uz = 1.0f - tz;
@@ -101,6 +133,11 @@ struct AdjArgs {
     float* grad_tsdf;    // res^3, accumulated (+=); may be null
     float* depth_out;    // W*H ray depth (0 on a miss); may be null
     double* partials;    // [slot][block]
+    // Empty-space skip bounds of this pose (brickSplat: per 2 x 2 pixel cell the nearest
+    // and farthest distance of a marked brick the cell's rays can meet), or null; the
+    // first pixel of each cell puts the cell back to its resting state after use.
+    unsigned* dsafe;
+    const int* skip_ok;
 };
 
 template <class S>
@@ -136,6 +173,20 @@ __global__ __launch_bounds__(256) void depthLossKernel(AdjVol<S> V, AdjArgs a) {
             t_near = smax(t_near, t0);
             t_far = smin(t_far, t1);
         }
+        // Lattice samples before t_safe lie in no marked brick, so all eight corners of
+        // their cells are far (unobserved, or tsdf >= 0.999): such a sample is invalid or
+        // positive, and the march passes it without stopping or hitting. Samples past
+        // t_end are far too, so no hit can follow them.
+        float t_safe = -__builtin_inff(), t_end = __builtin_inff();
+        const int cell = (y >> 1) * ((a.W + 1) >> 1) + (x >> 1);
+        if (a.dsafe) {
+            const uint2 d = *reinterpret_cast<const uint2*>(a.dsafe + 2 * (size_t)cell);
+            if (*a.skip_ok) {
+                const float dn = f3norm(dir);
+                t_safe = __uint_as_float(d.x) / dn;
+                t_end = __uint_as_float(d.y) / dn;
+            }
+        }
         float depth = 0.0f;
         if (t_near < t_far) {
             // Samples sit on the lattice t_k = t0 + k h anchored at the pose-independent
@@ -145,30 +196,47 @@ __global__ __launch_bounds__(256) void depthLossKernel(AdjVol<S> V, AdjArgs a) {
             const int k0 = t_near > t0 ? (int)ceilf((t_near - t0) / a.step) : 0;
             const int k1 = (int)floorf((t_far - t0) / a.step);
             int k = k0;
+            if (t0 + (float)k0 * a.step < t_safe && k0 < k1) {
+                // start at the last lattice sample before t_safe (its value enters the next
+                // crossing test); the ones before it are passed over unread
+                int ks = (int)floorf((t_safe - t0) / a.step);
+                ks = min(max(ks, k0), k1);
+                while (ks > k0 && !(t0 + (float)ks * a.step < t_safe)) --ks;
+                while (ks < k1 && t0 + (float)(ks + 1) * a.step < t_safe) ++ks;
+                k = ks;
+            }
             float t_prev = t0 + (float)k * a.step;
-            TriSample sp = sampleGrad<S>(V, f3add(cam, f3mul(dir, t_prev)));
-            bool go = k <= k1 && !(sp.valid && (!isfinite(sp.f) || sp.f < 0.0f));
+            // The march reads values only; the two samples of a bracket are read again
+            // with their gradients once it is found.
+            float fp = 0.0f;
+            bool vp = sampleValue<S>(V, f3add(cam, f3mul(dir, t_prev)), fp);
+            bool go = k <= k1 && !(vp && (!isfinite(fp) || fp < 0.0f));
             bool found = false;
             float t_b = 0.0f;
-            TriSample sb;
-            while (go && k < k1) {
+            while (go && k < k1 && t_prev < t_end) {
                 ++k;
                 const float t = t0 + (float)k * a.step;
-                const TriSample sc = sampleGrad<S>(V, f3add(cam, f3mul(dir, t)));
-                if (sc.valid && !isfinite(sc.f)) break;
-                if (sc.valid && sp.valid) {
-                    if (sp.f > 0.0f && sc.f <= 0.0f) {
+                float fc = 0.0f;
+                const bool vc = sampleValue<S>(V, f3add(cam, f3mul(dir, t)), fc);
+                if (vc && !isfinite(fc)) break;
+                if (vc && vp) {
+                    if (fp > 0.0f && fc <= 0.0f) {
                         found = true;
                         t_b = t;
-                        sb = sc;
                         break;
                     }
-                    if (sp.f < 0.0f && sc.f >= 0.0f) break;
-                } else if (sc.valid && sc.f < 0.0f) {
+                    if (fp < 0.0f && fc >= 0.0f) break;
+                } else if (vc && fc < 0.0f) {
                     break;
                 }
                 t_prev = t;
-                sp = sc;
+                fp = fc;
+                vp = vc;
+            }
+            TriSample sp, sb;
+            if (found) {
+                sp = sampleGrad<S>(V, f3add(cam, f3mul(dir, t_prev)));
+                sb = sampleGrad<S>(V, f3add(cam, f3mul(dir, t_b)));
             }
             const float d_obs = a.d_obs ? a.d_obs[y * a.W + x] : 0.0f;
             if (found) {
@@ -210,6 +278,8 @@ __global__ __launch_bounds__(256) void depthLossKernel(AdjVol<S> V, AdjArgs a) {
             }
         }
         if (a.depth_out) a.depth_out[y * a.W + x] = depth;
+        if (a.dsafe && ((x | y) & 1) == 0)
+            *reinterpret_cast<uint2*>(a.dsafe + 2 * (size_t)cell) = make_uint2(0x7f800000u, 0u);
     }
     // Fixed-order fold: wave butterfly, then the four waves in order.
 #pragma unroll
@@ -241,9 +311,14 @@ __global__ void depthLossFinish(const double* __restrict__ partials, int nb, dou
 
 int depthLossBlocks(int W, int H) { return ((W + 15) / 16) * ((H + 15) / 16); }
 
+hipError_t launchBrickSplat(const unsigned char* bricks, const VolParams& p, const PoseArg& pa, Intr K, int W, int H,
+                            unsigned* dsafe, int* ok, hipStream_t st);
+
+// bricks / dsafe / ok as for launchRaycast (null bricks: every lattice sample is read).
 hipError_t launchDepthLoss(int format, const void* vox, const VolParams& p, const PoseArg& pa, Intr K, int W, int H,
                            const float* d_obs, float step, int backward, float* grad_tsdf, float* depth_out,
-                           double* partials, double* out8, hipStream_t st) {
+                           double* partials, double* out8, const unsigned char* bricks, unsigned* dsafe, int* ok,
+                           hipStream_t st) {
     AdjArgs a;
     a.p = p;
     a.pa = pa;
@@ -256,6 +331,13 @@ hipError_t launchDepthLoss(int format, const void* vox, const VolParams& p, cons
     a.grad_tsdf = grad_tsdf;
     a.depth_out = depth_out;
     a.partials = partials;
+    a.dsafe = nullptr;
+    a.skip_ok = ok;
+    if (bricks && dsafe && ok) {
+        const hipError_t e = launchBrickSplat(bricks, p, pa, K, W, H, dsafe, ok, st);
+        if (e != hipSuccess) return e;
+        a.dsafe = dsafe;
+    }
     const int nb = depthLossBlocks(W, H);
     if (format == kFmtF32) {
         AdjVol<StoreF32> V{static_cast<const float2*>(vox), p.res, p.vs, 1.0f / p.vs, p.origin[0], p.origin[1],

@@ -54,7 +54,8 @@ hipError_t launchMarchingCubes(int format, const void* vox, const VolParams& p, 
                                McResult* res, hipStream_t st);
 hipError_t launchDepthLoss(int format, const void* vox, const VolParams& p, const PoseArg& pa, Intr K, int W, int H,
                            const float* d_obs, float step, int backward, float* grad_tsdf, float* depth_out,
-                           double* partials, double* out8, hipStream_t st);
+                           double* partials, double* out8, const unsigned char* bricks, unsigned* dsafe, int* ok,
+                           hipStream_t st);
 void icpTicksRead(const void* hostCopy, unsigned long long* ticks);
 hipError_t launchIcpIter(const float* lv, const float* ln, int W, int H, const float* mv, const float* mn, int MW,
                          int MH, Intr Km, float angle_cos, float dist_sq, int level, int iter, int prev_level,
@@ -750,6 +751,28 @@ int azu_integrate_noalloc(azu_volume* v, const float* d_depth_m, int W, int H, c
     return integrateImpl(v, d_depth_m, W, H, K, poseByValue(pose), min_depth, max_depth, s, sp);
 }
 
+// The skip bounds' scratch for a W x H image (grown with the image) and a current brick
+// map (rebuilt from the voxels once other writers may have changed them).
+static int prepareSkip(azu_volume* v, int W, int H, hipStream_t st) {
+    const size_t cells = (size_t)((W + 1) / 2) * (size_t)((H + 1) / 2);
+    if (cells > v->dsafe_cells) {
+        // grows only with the image; work queued on other streams may still use the old one
+        AZU_HIP_TRY(hipDeviceSynchronize(), "synchronise before skip scratch reallocation");
+        if (v->dsafe) (void)hipFree(v->dsafe);
+        v->dsafe = nullptr;
+        v->dsafe_cells = 0;
+        if (hipMalloc(&v->dsafe, (2 * cells + 1) * sizeof(unsigned)) != hipSuccess)
+            return fail(AZU_ERR_ALLOC, "raycast skip scratch allocation failed");
+        AZU_HIP_TRY(launchSkipInit(v->dsafe, cells, st), "skip scratch init");
+        v->dsafe_cells = cells;
+    }
+    if (v->skip && (v->external || v->bricks_stale)) {
+        AZU_HIP_TRY(launchBrickMap(v->format, v->vox, v->params.resolution, v->bricks, true, st), "brick map rebuild");
+        v->bricks_stale = false;
+    }
+    return AZU_OK;
+}
+
 static int raycastImpl(azu_volume* v, const PoseArg& pa, const float* K, int W, int H, float* dv, float* dn,
                        float* dt, azu_stream s) {
     if (!v || !dv || !dn) return fail(AZU_ERR_ARG, "null volume or output");
@@ -760,23 +783,8 @@ static int raycastImpl(azu_volume* v, const PoseArg& pa, const float* K, int W, 
     ProfScope ps(exact ? (v->format == AZU_FMT_F32 ? "raycast_exact_f32" : "raycast_exact_f16")
                        : (v->format == AZU_FMT_F32 ? "raycast_fast_f32" : "raycast_fast_f16"),
                  (hipStream_t)s);
-    const size_t cells = (size_t)((W + 1) / 2) * (size_t)((H + 1) / 2);
-    if (cells > v->dsafe_cells) {
-        // grows only with the image; work queued on other streams may still use the old one
-        AZU_HIP_TRY(hipDeviceSynchronize(), "synchronise before skip scratch reallocation");
-        if (v->dsafe) (void)hipFree(v->dsafe);
-        v->dsafe = nullptr;
-        v->dsafe_cells = 0;
-        if (hipMalloc(&v->dsafe, (2 * cells + 1) * sizeof(unsigned)) != hipSuccess)
-            return fail(AZU_ERR_ALLOC, "raycast skip scratch allocation failed");
-        AZU_HIP_TRY(launchSkipInit(v->dsafe, cells, (hipStream_t)s), "skip scratch init");
-        v->dsafe_cells = cells;
-    }
-    if (v->skip && (v->external || v->bricks_stale)) {
-        AZU_HIP_TRY(launchBrickMap(v->format, v->vox, v->params.resolution, v->bricks, true, (hipStream_t)s),
-                    "brick map rebuild");
-        v->bricks_stale = false;
-    }
+    const int rc = prepareSkip(v, W, H, (hipStream_t)s);
+    if (rc != AZU_OK) return rc;
     AZU_HIP_TRY(launchRaycast(v->format, exact, v->vox, toVol(v->params), v->dvs, v->skip ? v->bricks : nullptr,
                               v->dsafe,
                               reinterpret_cast<int*>(v->dsafe + 2 * v->dsafe_cells), pa, Intr{K[0], K[1], K[2], K[3]}, W,
@@ -818,9 +826,13 @@ int azu_raycast_depth_loss(azu_volume* v, const float pose[16], const float K[4]
         return fail(AZU_ERR_ALLOC, "adjoint output");
     {
         ProfScope ps(backward ? "depth_loss_bwd" : "depth_loss_fwd", st);
+        const int rc = prepareSkip(v, W, H, st);
+        if (rc != AZU_OK) return rc;
         AZU_HIP_TRY(launchDepthLoss(v->format, v->vox, toVol(v->params), poseByValue(pose),
                                     Intr{K[0], K[1], K[2], K[3]}, W, H, d_depth_obs, step_voxels * v->params.voxel_size,
-                                    backward, d_grad_tsdf, d_depth_out, v->adj_partials, v->adj_out, st),
+                                    backward, d_grad_tsdf, d_depth_out, v->adj_partials, v->adj_out,
+                                    v->skip ? v->bricks : nullptr, v->dsafe,
+                                    reinterpret_cast<int*>(v->dsafe + 2 * v->dsafe_cells), st),
                     "depth loss launch");
     }
     AZU_HIP_TRY(hipMemcpyAsync(out, v->adj_out, 8 * sizeof(double), hipMemcpyDeviceToHost, st), "loss read");

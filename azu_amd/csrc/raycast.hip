@@ -538,6 +538,15 @@ hipError_t launchSkipInit(unsigned* dsafe, size_t cells, hipStream_t st) {
     return hipGetLastError();
 }
 
+// The per-pixel skip bounds of a pose into dsafe / ok (brickSplat over the marked bricks).
+hipError_t launchBrickSplat(const unsigned char* bricks, const VolParams& p, const PoseArg& pa, Intr K, int W, int H,
+                            unsigned* dsafe, int* ok, hipStream_t st) {
+    const int nb = brickCount(p.res);
+    const unsigned blocks = (unsigned)(((size_t)nb * nb * nb + 3) / 4);
+    hipLaunchKernelGGL(brickSplat, dim3(blocks), dim3(256), 0, st, bricks, nb, p, pa, K, W, H, dsafe, ok);
+    return hipGetLastError();
+}
+
 // bricks: the volume's brick map, or null to take every sample; dsafe (two words per
 // 2 x 2 pixel cell, ((W + 1) / 2) ((H + 1) / 2) cells, 8-byte aligned, in the resting
 // state of launchSkipInit on entry and on return) and ok (one int): the skip bounds'
@@ -548,9 +557,8 @@ hipError_t launchRaycast(int format, bool exact, const void* vox, const VolParam
     const RayOut out{verts, norms, t_hit};
     RaySkip skip{nullptr, ok};
     if (bricks && dsafe && ok) {
-        const int nb = brickCount(p.res);
-        const unsigned blocks = (unsigned)(((size_t)nb * nb * nb + 3) / 4);
-        hipLaunchKernelGGL(brickSplat, dim3(blocks), dim3(256), 0, st, bricks, nb, p, pa, K, W, H, dsafe, ok);
+        const hipError_t e = launchBrickSplat(bricks, p, pa, K, W, H, dsafe, ok, st);
+        if (e != hipSuccess) return e;
         skip.dsafe = dsafe;
     }
     const bool far_origin = fabsf(p.origin[0]) >= 0x1p-60f && fabsf(p.origin[1]) >= 0x1p-60f &&

@@ -139,3 +139,21 @@ def test_tsdf_gradient_finite_differences_512(fused):
             assert abs(fd - g[i]) <= 0.05 * abs(g[i]) + 1e-2 * gmax, (i, fd, g[i])
     prob.set_voxels([], [])
     assert checked >= 8
+
+
+def test_skip_bounds_leave_the_loss_unchanged_512(fused):
+    """The march passes over the lattice samples the brick map's per-pixel bounds prove
+    far; with skipping off every sample is read. Ray depths, the loss, the contributing
+    pixels and the (fixed-order) pose gradient are bit-identical either way."""
+    prob = fused
+    vol = prob.vol
+    runs = {}
+    for on in (True, False):
+        vol.set_skip(on)
+        out, _ = prob.run(np.zeros(6), backward=True)
+        runs[on] = (out, prob.depth(np.zeros(6)))
+    vol.set_skip(True)
+    (a, da), (b, db) = runs[True], runs[False]
+    say("loss with / without skipping: %.12g / %.12g, pixels %d / %d" % (a[0], b[0], a[7], b[7]))
+    assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
+    assert np.array_equal(da.view(np.uint32), db.view(np.uint32))
