@@ -180,6 +180,22 @@ __device__ __forceinline__ bool boundsOver(const DepthBounds& b, int u0, int u1,
     return true;
 }
 
+// The same bounds from a finer level: the finest one on which the rectangle spans at
+// most kFineCells cells per axis (at most kFineCells^2 reads instead of 4), so the
+// bounds come from little more than the footprint itself.
+constexpr int kFineCells = 6;
+__device__ __forceinline__ bool boundsOverFine(const DepthBounds& b, int u0, int u1, int v0, int v1, float2& r) {
+    int l = 1;
+    while (l < kBoundLevels && (max((u1 >> l) - (u0 >> l), (v1 >> l) - (v0 >> l)) + 1) > kFineCells) ++l;
+    const int cx0 = u0 >> l, cx1 = u1 >> l, cy0 = v0 >> l, cy1 = v1 >> l;
+    if (cx1 - cx0 + 1 > kFineCells || cy1 - cy0 + 1 > kFineCells) return boundsOver(b, u0, u1, v0, v1, r);
+    const float2* L = b.lvl + b.off[l];
+    r = L[cy0 * b.w[l] + cx0];
+    for (int cy = cy0; cy <= cy1; ++cy)
+        for (int cx = cx0; cx <= cx1; ++cx) r = boundsJoin(r, L[cy * b.w[l] + cx]);
+    return true;
+}
+
 // Wave-uniform classification of the tile {xa..xb} x {y} x {za..zb} (a rectangle of
 // lattice points at fixed y), decided from its four corners: the rectangle's camera
 // coordinates are affine in (x, z), and the perspective image of a rectangle in front of
@@ -198,7 +214,7 @@ constexpr int kTileCull = 0, kTileExact = 1, kTileFree = 2;
 
 __device__ __forceinline__ int tileClass(const IntFrame& f, const VolParams& p, Intr K, int W, int H, float max_d,
                                          int xa, int xb, int y, int za, int zb, const DepthBounds& bnd,
-                                         float cbound) {
+                                         float cbound, bool fine = false) {
     float zmin = __builtin_inff(), zmax = -__builtin_inff();
     float umin = __builtin_inff(), umax = -__builtin_inff(), vmin = __builtin_inff(), vmax = -__builtin_inff();
     bool left = true, right = true, top = true, bottom = true, front = true;
@@ -240,7 +256,7 @@ __device__ __forceinline__ int tileClass(const IntFrame& f, const VolParams& p, 
     const int u1 = (int)fminf(u1f, (float)(W - 1)), v1 = (int)fminf(v1f, (float)(H - 1));
     if (u0 > u1 || v0 > v1) return kTileExact;
     float2 r;
-    if (!boundsOver(bnd, u0, u1, v0, v1, r)) return kTileExact;
+    if (!(fine ? boundsOverFine(bnd, u0, u1, v0, v1, r) : boundsOver(bnd, u0, u1, v0, v1, r))) return kTileExact;
     const float gap = p.trunc + 2.0f * eps + 1e-4f;
     if (r.y + gap < zmin) return kTileCull;                      // behind every in-band depth
     if (inside && r.x >= 0.0f && r.x - gap > zmax) return kTileFree;  // in front of every depth, all in band
@@ -281,7 +297,7 @@ __global__ __launch_bounds__(256) void tileClassKernel(VolParams p, Intr K, int 
     const float cbound = fabsf(f.t_wc.x) + fabsf(f.t_wc.y) + fabsf(f.t_wc.z) + fabsf(p.origin[0]) +
                          fabsf(p.origin[1]) + fabsf(p.origin[2]) + 3.0f * (float)p.res * p.vs;
     classes[tileIndex(p.res, xt, y, zb)] =
-        (unsigned char)tileClass(f, p, K, W, H, max_d, xt + js, xt + n - 1, y, za, ze, bnd, cbound);
+        (unsigned char)tileClass(f, p, K, W, H, max_d, xt + js, xt + n - 1, y, za, ze, bnd, cbound, true);
 }
 
 // One voxel word through a raw buffer resource (byte offsets: lane + uniform).
