@@ -15,11 +15,13 @@
 //    so that up to 8 waves a SIMD fit (64-row tiles allowed 4 and measured 390 against
 //    260 us when held to 2). Division and square root are IEEE-exact, nothing contracts.
 //
-//  * integrateFast<Store>: the FP16 + UInt16 storage form. The same tiles and tile
-//    classes, but lane = x column: the camera coordinate is an affine function of
-//    (x, z) evaluated with fmaf and the reciprocals are the hardware v_rcp_f32, so the
-//    lanes are independent voxels and each row of a tile is one coalesced access.
-//    Accuracy is stated against the FP64 oracle.
+//    integrateExact runs for both storage forms by default.
+//
+//  * integrateFast<Store>: the opt-in FAST mode, for either storage form. The same tiles
+//    and tile classes, but lane = x column: the camera coordinate is an affine function
+//    of (x, z) evaluated with fmaf and the reciprocals are the hardware v_rcp_f32, so
+//    the lanes are independent voxels and each row of a tile is one coalesced access.
+//    Not bit-identical to the reference; its accuracy is stated against the FP64 oracle.
 //
 // Both read the pose by value or from device memory (the ICP output), so a frame can
 // run with no host synchronisation. Updated / newly observed voxel counts are
@@ -543,7 +545,7 @@ __global__ __launch_bounds__(256, 8) void integrateExact(typename S::Word* __res
 }
 
 // ---------------------------------------------------------------------------
-// Fast kernel (the FP16 + UInt16 storage form; also selectable for FP32). The same
+// Fast kernel (the opt-in FAST mode, either storage form). The same
 // tiles and classes as integrateExact (cull / free / per-voxel), but a voxel's camera
 // coordinate is the affine c0 + x ax + z az evaluated with fmaf and the reciprocals
 // are the hardware's, so the lanes of a wave are independent voxels: lane = x column
