@@ -24,7 +24,7 @@ hipError_t launchIntegrate(int format, bool exact, void* vox, const VolParams& p
                            bool recip, void* bounds, unsigned long long* counters, unsigned char* bricks,
                            unsigned char* classes, hipStream_t st);
 size_t depthBoundsBytes(int W, int H);
-size_t tileClassBytes(int res);
+size_t exactScratchBytes(int res);
 hipError_t launchFillEmpty(int format, void* vox, size_t n, hipStream_t st);
 hipError_t launchBrickMap(int format, const void* vox, int res, unsigned char* bricks, bool scan, hipStream_t st);
 hipError_t launchRaycast(int format, bool exact, const void* vox, const VolParams& p, const FixedDiv& dvs,
@@ -252,7 +252,7 @@ struct azu_volume {
     bool recip = false;                                // verified Newton-Raphson 1 / x
     void* bounds = nullptr;                            // depth bounds of the last frame size
     size_t bounds_bytes = 0;
-    unsigned char* classes = nullptr;                  // exact integration's tile-class map
+    unsigned char* classes = nullptr;                  // exact integration's tile-class map + row carries
     size_t classes_bytes = 0;
     // Brick map for the raycast's empty-space skipping, kept by reset / upload /
     // integrate. Once the raw storage has been handed out, other writers may exist and
@@ -701,14 +701,14 @@ static int integrateImpl(azu_volume* v, const float* d_depth, int W, int H, cons
             if (hipMalloc(&v->bounds, need) != hipSuccess) return fail(AZU_ERR_ALLOC, "depth bounds allocation failed");
             v->bounds_bytes = need;
         }
-        const size_t cneed = tileClassBytes(v->params.resolution);
+        const size_t cneed = exactScratchBytes(v->params.resolution);
         if (exact && cneed > v->classes_bytes) {
-            AZU_HIP_TRY(hipDeviceSynchronize(), "synchronise before tile-class map reallocation");
+            AZU_HIP_TRY(hipDeviceSynchronize(), "synchronise before tile-class map / row carry reallocation");
             if (v->classes) (void)hipFree(v->classes);
             v->classes = nullptr;
             v->classes_bytes = 0;
             if (hipMalloc((void**)&v->classes, cneed) != hipSuccess)
-                return fail(AZU_ERR_ALLOC, "tile-class map allocation failed");
+                return fail(AZU_ERR_ALLOC, "tile-class map / row carry allocation failed");
             v->classes_bytes = cneed;
         }
     }

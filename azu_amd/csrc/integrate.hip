@@ -277,6 +277,15 @@ __host__ __device__ inline size_t tileIndex(int res, int xt, int y, int zb) {
 }
 size_t tileClassBytes(int res) { return (size_t)res * tileGrid(res) * tileGrid(res); }
 
+// Row carries of integrateExact: the camera coordinate of row (y, z) at the start of
+// row segment k >= 1 (x = xstart + k kSegCols), after the same serial x-steps from the
+// frustum box's first column that the segment's wave would otherwise take itself.
+// Entry ((k res + y) res + z), z fastest, so a wave's 32 rows read one 512-byte span.
+__host__ __device__ inline int carrySegs(int res) { return (res + kSegCols - 1) / kSegCols + 1; }
+static size_t carryOffset(int res) { return (tileClassBytes(res) + 255) & ~(size_t)255; }
+// The exact kernel's per-frame scratch: the tile-class map, then the row carries.
+size_t exactScratchBytes(int res) { return carryOffset(res) + (size_t)carrySegs(res) * res * res * sizeof(float4); }
+
 // One lane per tile of the frame's frustum box: tileClass with the arguments
 // integrateExact would pass, so the map holds the classes it would compute itself.
 __global__ __launch_bounds__(256) void tileClassKernel(VolParams p, Intr K, int W, int H, float max_d, PoseArg pa,
@@ -347,13 +356,48 @@ __device__ __forceinline__ void markTile(unsigned char* __restrict__ bricks, int
         }
 }
 
+// One lane per row (y, z) of the frame's frustum box in y and of its 32-row z groups in
+// z: the row's camera coordinate from the box's first column, stepped serially and stored
+// at every later segment start. The steps are integrateExact's own, in the same order,
+// so every wave of a segment k >= 1 starts from the coordinate it would have carried.
+__global__ __launch_bounds__(256) void rowCarryKernel(VolParams p, Intr K, int W, int H, float max_d, PoseArg pa,
+                                                      float4* __restrict__ carry) {
+    const IntFrame f = integrateSetup(p, pa, K, W, H, max_d);
+    if (!f.ok) return;
+    const int ny = f.y1 - f.y0 + 1;
+    const int zbase = f.z0 & ~(kTileRows - 1);
+    const int nz = min((f.z1 - zbase + kTileRows) / kTileRows * kTileRows, p.res - zbase);
+    const int xstart = f.x0 & ~(kChunk - 1);
+    const int nseg = (f.x1 - xstart + kSegCols) / kSegCols;
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (nseg < 2 || i >= (long long)ny * nz) return;
+    const int z = zbase + (int)(i % nz), y = f.y0 + (int)(i / nz);
+    const float vs = p.vs;
+    const F3 pp{p.origin[0] + (float)f.x0 * vs, p.origin[1] + (float)y * vs, p.origin[2] + (float)z * vs};
+    F3 c = f3add(mulv(f.R_wc, pp), f.t_wc);
+    // segment 1 starts at most kSegCols steps past x0, every later one kSegCols further
+    int x = f.x0;
+    for (int k = 1; k < nseg; ++k) {
+        const int xs = xstart + k * kSegCols;
+        int n = xs - x;
+        for (; n >= 8; n -= 8) {
+#pragma unroll
+            for (int u = 0; u < 8; ++u) c = f3add(c, f.step);
+        }
+        for (; n > 0; --n) c = f3add(c, f.step);
+        x = xs;
+        carry[((size_t)k * p.res + y) * p.res + z] = make_float4(c.x, c.y, c.z, 0.0f);
+    }
+}
+
 template <class S, int FD, bool RC>
 __global__ __launch_bounds__(256, 8) void integrateExact(typename S::Word* __restrict__ vox, VolParams p, Intr K,
                                                       int W, int H, const float* __restrict__ depth, float min_d,
                                                       float max_d, PoseArg pa, FixedDiv dtr, DepthBounds bnd,
                                                       unsigned long long* __restrict__ counters,
                                                       unsigned char* __restrict__ bricks, int nb,
-                                                      const unsigned char* __restrict__ classes) {
+                                                      const unsigned char* __restrict__ classes,
+                                                      const float4* __restrict__ carry) {
     __shared__ float lds[4][kTileRows * kLdsStride];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const IntFrame f = integrateSetup(p, pa, K, W, H, max_d);
@@ -364,8 +408,9 @@ __global__ __launch_bounds__(256, 8) void integrateExact(typename S::Word* __res
     // idle. The aligned groups also measure faster than groups starting at the box (FP16
     // 290 against 347 us at 512^3 on the same frames, with 64-row tiles).
     // A row is cut into segments of kSegChunks tiles, one wave each, so that a frame has
-    // several waves per SIMD to hide its gathers behind; a wave first carries its rows'
-    // camera coordinates to its segment by the same serial x-steps the whole row takes.
+    // several waves per SIMD to hide its gathers behind; a wave starts its rows' camera
+    // coordinates at its segment from the row carries (rowCarryKernel), or without them
+    // carries them there itself by the same serial x-steps the whole row takes.
     const int ny = f.y1 - f.y0 + 1;
     const int zbase = f.z0 & ~(kTileRows - 1);
     const int nzg = (f.z1 - zbase + kTileRows) / kTileRows;
@@ -383,9 +428,15 @@ __global__ __launch_bounds__(256, 8) void integrateExact(typename S::Word* __res
     const int z = zb + r;
     const bool row_ok = z >= f.z0 && z <= f.z1;
     const float vs = p.vs, trunc = p.trunc;
-    const F3 pp{p.origin[0] + (float)f.x0 * vs, p.origin[1] + (float)y * vs, p.origin[2] + (float)z * vs};
-    F3 c = f3add(mulv(f.R_wc, pp), f.t_wc);
-    for (int x = f.x0; x < xs; ++x) c = f3add(c, f.step);
+    F3 c;
+    if (carry && seg > 0 && z < p.res) {
+        const float4 q = carry[((size_t)seg * p.res + y) * p.res + z];
+        c = F3{q.x, q.y, q.z};
+    } else {
+        const F3 pp{p.origin[0] + (float)f.x0 * vs, p.origin[1] + (float)y * vs, p.origin[2] + (float)z * vs};
+        c = f3add(mulv(f.R_wc, pp), f.t_wc);
+        for (int x = f.x0; x < xs; ++x) c = f3add(c, f.step);
+    }
     float* L = lds[wv];
     const size_t res = (size_t)p.res, plane = res * res;
     const float fW = (float)W, fH = (float)H;
@@ -823,7 +874,8 @@ static DepthBounds boundsFor(void* bounds, const float* depth, int W, int H, flo
 }
 
 // bricks: the brick map to keep (brickCount(res)^3 bytes), or null. classes: the exact
-// kernel's tile-class map (tileClassBytes(res)), or null to classify inside the kernel.
+// kernel's per-frame scratch (exactScratchBytes(res): tile-class map and row carries), or
+// null to classify and carry inside the kernel.
 hipError_t launchIntegrate(int format, bool exact, void* vox, const VolParams& p, Intr K, int W, int H,
                            const float* depth, float min_d, float max_d, const PoseArg& pa, const FixedDiv& dtr,
                            bool recip, void* bounds, unsigned long long* counters, unsigned char* bricks,
@@ -840,15 +892,23 @@ hipError_t launchIntegrate(int format, bool exact, void* vox, const VolParams& p
         // 2^-100 and the unguarded fixed-divisor form runs.
         const bool fd = dtr.fast != 0, rc = recip, far = min_d >= 0x1p-60f;
         const unsigned char* cmap = nullptr;
+        const float4* carry = nullptr;
         if (classes && bnd.lvl) {
             const long long tiles = (long long)p.res * tileGrid(p.res) * tileGrid(p.res);
             hipLaunchKernelGGL(tileClassKernel, dim3((unsigned)((tiles + 255) / 256)), dim3(256), 0, st, p, K, W, H,
                                max_d, pa, bnd, classes);
             cmap = classes;
         }
+        if (classes) {
+            float4* rows = reinterpret_cast<float4*>(classes + carryOffset(p.res));
+            const long long n = (long long)p.res * p.res;
+            hipLaunchKernelGGL(rowCarryKernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, K, W, H, max_d,
+                               pa, rows);
+            carry = rows;
+        }
 #define AZU_INT_EXACT(S, W, FD, RC)                                                                        \
     hipLaunchKernelGGL((integrateExact<S, FD, RC>), grid, block, 0, st, (W*)vox, p, K, W_, H, depth, min_d, \
-                       max_d, pa, dtr, bnd, counters, bricks, nb, cmap)
+                       max_d, pa, dtr, bnd, counters, bricks, nb, cmap, carry)
         const int W_ = W;
         if (format == kFmtF32) {
             if (fd && far && rc) AZU_INT_EXACT(StoreF32, float2, 2, true);
