@@ -358,9 +358,12 @@ __device__ __forceinline__ void markTile(unsigned char* __restrict__ bricks, int
 
 // One lane per row (y, z) of the frame's frustum box in y and of its 32-row z groups in
 // z: the row's camera coordinate from the box's first column, stepped serially and stored
-// at every later segment start. The steps are integrateExact's own, in the same order,
-// so every wave of a segment k >= 1 starts from the coordinate it would have carried.
+// at the start of every later segment that holds an exact tile (every later segment
+// without a class map). The steps are integrateExact's own, in the same order, so every
+// wave of a segment k >= 1 with an exact tile starts from the coordinate it would have
+// carried; a wave whose segment holds none never uses its coordinate.
 __global__ __launch_bounds__(256) void rowCarryKernel(VolParams p, Intr K, int W, int H, float max_d, PoseArg pa,
+                                                      const unsigned char* __restrict__ classes,
                                                       float4* __restrict__ carry) {
     const IntFrame f = integrateSetup(p, pa, K, W, H, max_d);
     if (!f.ok) return;
@@ -372,12 +375,27 @@ __global__ __launch_bounds__(256) void rowCarryKernel(VolParams p, Intr K, int W
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
     if (nseg < 2 || i >= (long long)ny * nz) return;
     const int z = zbase + (int)(i % nz), y = f.y0 + (int)(i / nz);
+    // With the class map: only segments holding an exact tile need their carry, and no
+    // step past the last of them is taken.
+    int last = nseg - 1;
+    unsigned need = ~0u;
+    if (classes && nseg <= 31) {
+        const int zb = z & ~(kTileRows - 1);
+        need = 0u;
+        for (int k = 1; k < nseg; ++k)
+#pragma unroll
+            for (int t = 0; t < kSegChunks; ++t) {
+                const int xt = xstart + k * kSegCols + t * kChunk;
+                if (xt <= f.x1 && classes[tileIndex(p.res, xt, y, zb)] == kTileExact) need |= 1u << k;
+            }
+        last = need ? 31 - __builtin_clz(need) : 0;
+    }
     const float vs = p.vs;
     const F3 pp{p.origin[0] + (float)f.x0 * vs, p.origin[1] + (float)y * vs, p.origin[2] + (float)z * vs};
     F3 c = f3add(mulv(f.R_wc, pp), f.t_wc);
     // segment 1 starts at most kSegCols steps past x0, every later one kSegCols further
     int x = f.x0;
-    for (int k = 1; k < nseg; ++k) {
+    for (int k = 1; k <= last; ++k) {
         const int xs = xstart + k * kSegCols;
         int n = xs - x;
         for (; n >= 8; n -= 8) {
@@ -386,7 +404,7 @@ __global__ __launch_bounds__(256) void rowCarryKernel(VolParams p, Intr K, int W
         }
         for (; n > 0; --n) c = f3add(c, f.step);
         x = xs;
-        carry[((size_t)k * p.res + y) * p.res + z] = make_float4(c.x, c.y, c.z, 0.0f);
+        if ((need >> k) & 1u) carry[((size_t)k * p.res + y) * p.res + z] = make_float4(c.x, c.y, c.z, 0.0f);
     }
 }
 
@@ -903,7 +921,7 @@ hipError_t launchIntegrate(int format, bool exact, void* vox, const VolParams& p
             float4* rows = reinterpret_cast<float4*>(classes + carryOffset(p.res));
             const long long n = (long long)p.res * p.res;
             hipLaunchKernelGGL(rowCarryKernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, K, W, H, max_d,
-                               pa, rows);
+                               pa, cmap, rows);
             carry = rows;
         }
 #define AZU_INT_EXACT(S, W, FD, RC)                                                                        \
