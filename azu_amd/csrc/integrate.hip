@@ -82,6 +82,34 @@ __device__ __forceinline__ IntFrame integrateSetup(const VolParams& p, const Pos
     return f;
 }
 
+// Item pipeline constants and the frame's shared record (see the item pipeline below).
+constexpr int kSubRows = 8;      // z rows per sub-tile
+constexpr int kSlots = 256;      // work-list slots
+constexpr int kGroupItems = 4;   // sub-tiles per wave step (4 x kSubRows = 32 phase-1 rows)
+constexpr int kItemBlocks = 2048;  // integrateItems grid: 8192 waves, 32 per slot
+static_assert(kGroupItems * kSubRows == 32, "a group fills the 32 phase-1 rows of a wave");
+static_assert(kBrick % kSubRows == 0, "a sub-tile lies in one z-brick");
+
+struct IntShared {
+    IntFrame f;
+    float cbound;           // bound on every camera coordinate's magnitude (tileClass's eps)
+    // list lengths per slot, one word each so that a wave appends to both lists with one
+    // atomic: exact in the low half, free in the high half
+    unsigned long long cnt[kSlots];
+};
+
+// The frame's shared record (one lane) and the list lengths (cleared by the block).
+__device__ __forceinline__ void itemSetup(IntShared* __restrict__ sh, const VolParams& p, const PoseArg& pa, Intr K,
+                                          int W, int H, float max_d) {
+    for (int i = threadIdx.x; i < kSlots; i += blockDim.x) sh->cnt[i] = 0ull;
+    if (threadIdx.x == 0) {
+        const IntFrame f = integrateSetup(p, pa, K, W, H, max_d);
+        sh->f = f;
+        sh->cbound = fabsf(f.t_wc.x) + fabsf(f.t_wc.y) + fabsf(f.t_wc.z) + fabsf(p.origin[0]) + fabsf(p.origin[1]) +
+                     fabsf(p.origin[2]) + 3.0f * (float)p.res * p.vs;
+    }
+}
+
 // Updated / newly-observed voxel counts. Each wave keeps its counts in registers and
 // adds them once, at its end, to one of kCounterSlots slot pairs (chosen by block), so
 // no address sees more than a few hundred atomics per frame; readers sum the slots.
@@ -121,8 +149,12 @@ __device__ __forceinline__ float2 boundsJoin(float2 a, float2 b) {
 }
 
 // One block per 64 x 64 pixel region: levels 1..6 of its cells, reduced in LDS.
+// Block (0, 0) also writes the frame's shared record of the item pipeline when sh is set.
 __global__ __launch_bounds__(256) void depthBoundsKernel(const float* __restrict__ depth, int W, int H, float min_d,
-                                                         float max_d, float2* __restrict__ out, DepthBounds b) {
+                                                         float max_d, float2* __restrict__ out, DepthBounds b,
+                                                         IntShared* __restrict__ sh, VolParams p, PoseArg pa,
+                                                         Intr K) {
+    if (sh && blockIdx.x == 0 && blockIdx.y == 0) itemSetup(sh, p, pa, K, W, H, max_d);
     __shared__ float2 cell[32][33];
     const int tid = threadIdx.x;
     const int X0 = blockIdx.x * 64, Y0 = blockIdx.y * 64;
@@ -192,9 +224,17 @@ __device__ __forceinline__ bool boundsOverFine(const DepthBounds& b, int u0, int
     const int cx0 = u0 >> l, cx1 = u1 >> l, cy0 = v0 >> l, cy1 = v1 >> l;
     if (cx1 - cx0 + 1 > kFineCells || cy1 - cy0 + 1 > kFineCells) return boundsOver(b, u0, u1, v0, v1, r);
     const float2* L = b.lvl + b.off[l];
-    r = L[cy0 * b.w[l] + cx0];
-    for (int cy = cy0; cy <= cy1; ++cy)
-        for (int cx = cx0; cx <= cx1; ++cx) r = boundsJoin(r, L[cy * b.w[l] + cx]);
+    const int w = b.w[l];
+    // A row of cells at a time, its kFineCells reads issued before the first join (the
+    // cells past the rectangle's last column repeat it; the join is idempotent).
+    r = L[cy0 * w + cx0];
+    for (int cy = cy0; cy <= cy1; ++cy) {
+        float2 c[kFineCells];
+#pragma unroll
+        for (int i = 0; i < kFineCells; ++i) c[i] = L[cy * w + min(cx0 + i, cx1)];
+#pragma unroll
+        for (int i = 0; i < kFineCells; ++i) r = boundsJoin(r, c[i]);
+    }
     return true;
 }
 
@@ -267,49 +307,7 @@ __device__ __forceinline__ int tileClass(const IntFrame& f, const VolParams& p, 
 
 static_assert(kChunk == 2 * kBrick, "a tile spans two bricks in x");
 
-// Class map of integrateExact's tiles: one byte per 32-column x kTileRows-row tile at
-// one y, indexed by the tile's absolute position; a frame writes the entries of its
-// frustum box, and only those are read.
 __host__ __device__ inline int tileGrid(int res) { return (res + kChunk - 1) / kChunk; }
-__host__ __device__ inline size_t tileIndex(int res, int xt, int y, int zb) {
-    const size_t g = (size_t)tileGrid(res);
-    return ((size_t)y * g + (size_t)(zb / kTileRows)) * g + (size_t)(xt / kChunk);
-}
-size_t tileClassBytes(int res) { return (size_t)res * tileGrid(res) * tileGrid(res); }
-
-// Row carries of integrateExact: the camera coordinate of row (y, z) at the start of
-// row segment k >= 1 (x = xstart + k kSegCols), after the same serial x-steps from the
-// frustum box's first column that the segment's wave would otherwise take itself.
-// Entry ((k res + y) res + z), z fastest, so a wave's 32 rows read one 512-byte span.
-__host__ __device__ inline int carrySegs(int res) { return (res + kSegCols - 1) / kSegCols + 1; }
-static size_t carryOffset(int res) { return (tileClassBytes(res) + 255) & ~(size_t)255; }
-// The exact kernel's per-frame scratch: the tile-class map, then the row carries.
-size_t exactScratchBytes(int res) { return carryOffset(res) + (size_t)carrySegs(res) * res * res * sizeof(float4); }
-
-// One lane per tile of the frame's frustum box: tileClass with the arguments
-// integrateExact would pass, so the map holds the classes it would compute itself.
-__global__ __launch_bounds__(256) void tileClassKernel(VolParams p, Intr K, int W, int H, float max_d, PoseArg pa,
-                                                       DepthBounds bnd, unsigned char* __restrict__ classes) {
-    const IntFrame f = integrateSetup(p, pa, K, W, H, max_d);
-    if (!f.ok) return;
-    const int ny = f.y1 - f.y0 + 1;
-    const int zbase = f.z0 & ~(kTileRows - 1);
-    const int nzg = (f.z1 - zbase + kTileRows) / kTileRows;
-    const int xstart = f.x0 & ~(kChunk - 1);
-    const int ntx = (f.x1 - xstart + kChunk) / kChunk;
-    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (i >= (long long)ny * nzg * ntx) return;
-    const int xt = xstart + (int)(i % ntx) * kChunk;
-    const long long rest = i / ntx;
-    const int y = f.y0 + (int)(rest % ny);
-    const int zb = zbase + (int)(rest / ny) * kTileRows;
-    const int za = max(zb, f.z0), ze = min(zb + kTileRows - 1, f.z1);
-    const int js = max(f.x0 - xt, 0), n = min(kChunk, f.x1 - xt + 1);
-    const float cbound = fabsf(f.t_wc.x) + fabsf(f.t_wc.y) + fabsf(f.t_wc.z) + fabsf(p.origin[0]) +
-                         fabsf(p.origin[1]) + fabsf(p.origin[2]) + 3.0f * (float)p.res * p.vs;
-    classes[tileIndex(p.res, xt, y, zb)] =
-        (unsigned char)tileClass(f, p, K, W, H, max_d, xt + js, xt + n - 1, y, za, ze, bnd, cbound, true);
-}
 
 // One voxel word through a raw buffer resource (byte offsets: lane + uniform).
 template <class S> struct BufWord;
@@ -356,66 +354,12 @@ __device__ __forceinline__ void markTile(unsigned char* __restrict__ bricks, int
         }
 }
 
-// One lane per row (y, z) of the frame's frustum box in y and of its 32-row z groups in
-// z: the row's camera coordinate from the box's first column, stepped serially and stored
-// at the start of every later segment that holds an exact tile (every later segment
-// without a class map). The steps are integrateExact's own, in the same order, so every
-// wave of a segment k >= 1 with an exact tile starts from the coordinate it would have
-// carried; a wave whose segment holds none never uses its coordinate.
-__global__ __launch_bounds__(256) void rowCarryKernel(VolParams p, Intr K, int W, int H, float max_d, PoseArg pa,
-                                                      const unsigned char* __restrict__ classes,
-                                                      float4* __restrict__ carry) {
-    const IntFrame f = integrateSetup(p, pa, K, W, H, max_d);
-    if (!f.ok) return;
-    const int ny = f.y1 - f.y0 + 1;
-    const int zbase = f.z0 & ~(kTileRows - 1);
-    const int nz = min((f.z1 - zbase + kTileRows) / kTileRows * kTileRows, p.res - zbase);
-    const int xstart = f.x0 & ~(kChunk - 1);
-    const int nseg = (f.x1 - xstart + kSegCols) / kSegCols;
-    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (nseg < 2 || i >= (long long)ny * nz) return;
-    const int z = zbase + (int)(i % nz), y = f.y0 + (int)(i / nz);
-    // With the class map: only segments holding an exact tile need their carry, and no
-    // step past the last of them is taken.
-    int last = nseg - 1;
-    unsigned need = ~0u;
-    if (classes && nseg <= 31) {
-        const int zb = z & ~(kTileRows - 1);
-        need = 0u;
-        for (int k = 1; k < nseg; ++k)
-#pragma unroll
-            for (int t = 0; t < kSegChunks; ++t) {
-                const int xt = xstart + k * kSegCols + t * kChunk;
-                if (xt <= f.x1 && classes[tileIndex(p.res, xt, y, zb)] == kTileExact) need |= 1u << k;
-            }
-        last = need ? 31 - __builtin_clz(need) : 0;
-    }
-    const float vs = p.vs;
-    const F3 pp{p.origin[0] + (float)f.x0 * vs, p.origin[1] + (float)y * vs, p.origin[2] + (float)z * vs};
-    F3 c = f3add(mulv(f.R_wc, pp), f.t_wc);
-    // segment 1 starts at most kSegCols steps past x0, every later one kSegCols further
-    int x = f.x0;
-    for (int k = 1; k <= last; ++k) {
-        const int xs = xstart + k * kSegCols;
-        int n = xs - x;
-        for (; n >= 8; n -= 8) {
-#pragma unroll
-            for (int u = 0; u < 8; ++u) c = f3add(c, f.step);
-        }
-        for (; n > 0; --n) c = f3add(c, f.step);
-        x = xs;
-        if ((need >> k) & 1u) carry[((size_t)k * p.res + y) * p.res + z] = make_float4(c.x, c.y, c.z, 0.0f);
-    }
-}
-
 template <class S, int FD, bool RC>
 __global__ __launch_bounds__(256, 8) void integrateExact(typename S::Word* __restrict__ vox, VolParams p, Intr K,
                                                       int W, int H, const float* __restrict__ depth, float min_d,
                                                       float max_d, PoseArg pa, FixedDiv dtr, DepthBounds bnd,
                                                       unsigned long long* __restrict__ counters,
-                                                      unsigned char* __restrict__ bricks, int nb,
-                                                      const unsigned char* __restrict__ classes,
-                                                      const float4* __restrict__ carry) {
+                                                      unsigned char* __restrict__ bricks, int nb) {
     __shared__ float lds[4][kTileRows * kLdsStride];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const IntFrame f = integrateSetup(p, pa, K, W, H, max_d);
@@ -426,9 +370,8 @@ __global__ __launch_bounds__(256, 8) void integrateExact(typename S::Word* __res
     // idle. The aligned groups also measure faster than groups starting at the box (FP16
     // 290 against 347 us at 512^3 on the same frames, with 64-row tiles).
     // A row is cut into segments of kSegChunks tiles, one wave each, so that a frame has
-    // several waves per SIMD to hide its gathers behind; a wave starts its rows' camera
-    // coordinates at its segment from the row carries (rowCarryKernel), or without them
-    // carries them there itself by the same serial x-steps the whole row takes.
+    // several waves per SIMD to hide its gathers behind; a wave carries its rows' camera
+    // coordinates to its segment by the same serial x-steps the whole row takes.
     const int ny = f.y1 - f.y0 + 1;
     const int zbase = f.z0 & ~(kTileRows - 1);
     const int nzg = (f.z1 - zbase + kTileRows) / kTileRows;
@@ -446,15 +389,9 @@ __global__ __launch_bounds__(256, 8) void integrateExact(typename S::Word* __res
     const int z = zb + r;
     const bool row_ok = z >= f.z0 && z <= f.z1;
     const float vs = p.vs, trunc = p.trunc;
-    F3 c;
-    if (carry && seg > 0 && z < p.res) {
-        const float4 q = carry[((size_t)seg * p.res + y) * p.res + z];
-        c = F3{q.x, q.y, q.z};
-    } else {
-        const F3 pp{p.origin[0] + (float)f.x0 * vs, p.origin[1] + (float)y * vs, p.origin[2] + (float)z * vs};
-        c = f3add(mulv(f.R_wc, pp), f.t_wc);
-        for (int x = f.x0; x < xs; ++x) c = f3add(c, f.step);
-    }
+    const F3 pp{p.origin[0] + (float)f.x0 * vs, p.origin[1] + (float)y * vs, p.origin[2] + (float)z * vs};
+    F3 c = f3add(mulv(f.R_wc, pp), f.t_wc);
+    for (int x = f.x0; x < xs; ++x) c = f3add(c, f.step);
     float* L = lds[wv];
     const size_t res = (size_t)p.res, plane = res * res;
     const float fW = (float)W, fH = (float)H;
@@ -463,20 +400,15 @@ __global__ __launch_bounds__(256, 8) void integrateExact(typename S::Word* __res
     const float cbound = fabsf(f.t_wc.x) + fabsf(f.t_wc.y) + fabsf(f.t_wc.z) +
                          fabsf(p.origin[0]) + fabsf(p.origin[1]) + fabsf(p.origin[2]) + 3.0f * (float)p.res * vs;
 
-    // The classes of the segment's tiles, lane t taking tile t: read from the frame's
-    // class map (tileClassKernel), or, without one, classified here, all tiles at once so
-    // that their depth-bounds reads overlap. They come back as two wave masks.
+    // The classes of the segment's tiles, lane t taking tile t, all tiles at once so that
+    // their depth-bounds reads overlap. They come back as two wave masks.
     unsigned long long cls_lo, cls_hi;
     {
         int my = kTileCull;
         const int xt = xs + lane * kChunk;
         if (lane < kSegChunks && xt <= xe) {
-            if (classes) {
-                my = classes[tileIndex(p.res, xt, y, zb)];
-            } else {
-                const int js = max(f.x0 - xt, 0), n = min(kChunk, f.x1 - xt + 1);
-                my = tileClass(f, p, K, W, H, max_d, xt + js, xt + n - 1, y, za, ze, bnd, cbound);
-            }
+            const int js = max(f.x0 - xt, 0), n = min(kChunk, f.x1 - xt + 1);
+            my = tileClass(f, p, K, W, H, max_d, xt + js, xt + n - 1, y, za, ze, bnd, cbound);
         }
         cls_lo = __ballot((my & 1) != 0);
         cls_hi = __ballot((my & 2) != 0);
@@ -600,6 +532,350 @@ __global__ __launch_bounds__(256, 8) void integrateExact(typename S::Word* __res
         }
         if (bricks && lane == 0 && (near0 | near1) != 0)
             markTile(bricks, nb, xc / kBrick, y, zb / kBrick, near0, near1, shared0, shared1);
+        waveSync();
+    }
+    if (counters) {
+        unsigned long long a = n_upd, b = n_new;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            a += __shfl_xor(a, o);
+            b += __shfl_xor(b, o);
+        }
+        flushCounts(counters, a, b);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Item pipeline of the exact integration (the default when the volume has its scratch).
+// The same per-voxel arithmetic as integrateExact, bit for bit, organised so that only
+// the voxels that can update are visited:
+//   * integrateSetup runs once per frame (depthBoundsKernel's first lane) into an
+//     IntShared record that every later kernel of the frame reads;
+//   * subtileClassKernel classifies sub-tiles of 32 columns x kSubRows rows at one y (a
+//     quarter of a tile's rows: a tighter depth range per unit, so more of them are free
+//     or culled) and appends every exact and every free one to a work list;
+//   * itemCarryKernel walks each row of the frustum box by the serial x-steps and stores
+//     the row's camera coordinate at the first column of every exact sub-tile it holds;
+//   * integrateItems: a persistent grid whose waves take groups of kGroupItems sub-tiles
+//     from the lists; exact groups run integrateExact's phase 1 (projection, gather,
+//     gates into LDS) from the stored coordinates, then every group folds its voxels as
+//     integrateExact's phase 2 (free sub-tiles with tsdf_new = 1, no projection).
+// The lists are split into kSlots slots (a class block appends to slot blockIdx % kSlots),
+// so that no list counter takes more than a few dozen atomics a frame; the order of the
+// items inside a list is whatever the atomics give, and the result does not depend on it
+// (every voxel belongs to one item; counts are sums; brick marks are idempotent).
+// ---------------------------------------------------------------------------
+__host__ __device__ inline int subGroups(int res) { return (res + kSubRows - 1) / kSubRows; }
+__host__ __device__ inline size_t subCount(int res) { return (size_t)res * subGroups(res) * tileGrid(res); }
+__host__ __device__ inline unsigned classBlocks(int res) { return (unsigned)((subCount(res) + 255) / 256); }
+// Capacity of one slot's list: every sub-tile of the class blocks that append to it.
+__host__ __device__ inline unsigned slotCap(int res) { return (classBlocks(res) + kSlots - 1) / kSlots * 256u; }
+
+// Scratch of the item pipeline: the shared record, the exact-item index of every sub-tile
+// (-1: not exact), the two lists, and kSubRows row coordinates per exact item.
+struct ItemScratch {
+    IntShared* sh;
+    int* map;
+    unsigned* lists;  // [2][kSlots][cap]
+    float4* carry;    // [kSlots * cap][kSubRows]
+    unsigned cap;
+};
+static size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+static ItemScratch itemScratch(void* base, int res) {
+    ItemScratch s;
+    unsigned char* b = static_cast<unsigned char*>(base);
+    s.cap = slotCap(res);
+    const size_t nl = (size_t)kSlots * s.cap;
+    s.sh = reinterpret_cast<IntShared*>(b);
+    b += align256(sizeof(IntShared));
+    s.map = reinterpret_cast<int*>(b);
+    b += align256(subCount(res) * sizeof(int));
+    s.lists = reinterpret_cast<unsigned*>(b);
+    b += align256(2 * nl * sizeof(unsigned));
+    s.carry = reinterpret_cast<float4*>(b);
+    return s;
+}
+size_t exactScratchBytes(int res) {
+    const size_t nl = (size_t)kSlots * slotCap(res);
+    return align256(sizeof(IntShared)) + align256(subCount(res) * sizeof(int)) + align256(2 * nl * sizeof(unsigned)) +
+           nl * kSubRows * sizeof(float4);
+}
+
+// A list entry: the sub-tile's y, z group and x tile.
+__device__ __forceinline__ unsigned itemPack(int y, int zg, int xi) {
+    return ((unsigned)y << 14) | ((unsigned)zg << 6) | (unsigned)xi;
+}
+__device__ __forceinline__ void itemUnpack(unsigned e, int& y, int& zg, int& xt) {
+    xt = (int)(e & 63u) * kChunk;
+    zg = (int)((e >> 6) & 255u);
+    y = (int)(e >> 14);
+}
+
+// One lane per sub-tile of the volume (lanes outside the frame's frustum box take none):
+// tileClass over the sub-tile's rows inside the box, then the appends.
+__global__ __launch_bounds__(256) void subtileClassKernel(VolParams p, Intr K, int W, int H, float max_d,
+                                                          DepthBounds bnd, ItemScratch is) {
+    const IntFrame f = is.sh->f;
+    const float cbound = is.sh->cbound;
+    const int lane = threadIdx.x & 63;
+    const int nzg = subGroups(p.res), ntx = tileGrid(p.res);
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    int cls = kTileCull, y = 0, zg = 0, xi = 0;
+    bool inbox = false;
+    if (f.ok && i < (long long)subCount(p.res)) {
+        xi = (int)(i % ntx);
+        const long long rest = i / ntx;
+        zg = (int)(rest % nzg);
+        y = (int)(rest / nzg);
+        const int xt = xi * kChunk, zb = zg * kSubRows;
+        const int za = max(zb, f.z0), ze = min(zb + kSubRows - 1, f.z1);
+        const int xa = max(xt, f.x0), xe = min(xt + kChunk - 1, f.x1);
+        inbox = y >= f.y0 && y <= f.y1 && za <= ze && xa <= xe;
+        if (inbox) cls = tileClass(f, p, K, W, H, max_d, xa, xe, y, za, ze, bnd, cbound, true);
+    }
+    const int slot = (int)(blockIdx.x % kSlots);
+    const unsigned long long me = __ballot(cls == kTileExact), mf = __ballot(cls == kTileFree);
+    if ((me | mf) != 0ull) {
+        unsigned long long base = 0ull;
+        if (lane == 0)
+            base = atomicAdd(&is.sh->cnt[slot], (unsigned long long)__popcll(me) |
+                                                    ((unsigned long long)__popcll(mf) << 32));
+        base = __shfl(base, 0);
+        const bool want = cls == kTileExact || cls == kTileFree;
+        if (want) {
+            const int l = cls == kTileExact ? 0 : 1;
+            const unsigned long long m = l == 0 ? me : mf;
+            const unsigned pos = (l == 0 ? (unsigned)base : (unsigned)(base >> 32)) +
+                                 __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+            is.lists[((size_t)l * kSlots + slot) * is.cap + pos] = itemPack(y, zg, xi);
+            if (l == 0) is.map[i] = (int)((unsigned)slot * is.cap + pos);
+        }
+    }
+    if (inbox && cls != kTileExact) is.map[i] = -1;
+}
+
+// One lane per row (y, z) of the frustum box: the row's camera coordinate from the box's
+// first column, stepped serially (integrateExact's own steps, in its order), stored at
+// the first in-box column of every exact sub-tile of the row; no step is taken past the
+// last of them.
+__global__ __launch_bounds__(256) void itemCarryKernel(VolParams p, ItemScratch is) {
+    const IntFrame f = is.sh->f;
+    if (!f.ok) return;
+    const int ny = f.y1 - f.y0 + 1, nz = f.z1 - f.z0 + 1;
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (long long)ny * nz) return;
+    const int z = f.z0 + (int)(i % nz), y = f.y0 + (int)(i / nz);
+    const int ntx = tileGrid(p.res);
+    const int t0 = f.x0 / kChunk, t1 = f.x1 / kChunk;  // t1 - t0 < 64 (res <= 2048)
+    const int* row = is.map + ((size_t)y * subGroups(p.res) + z / kSubRows) * ntx;
+    // The row's tiles 16 at a time, their map entries read together: the coordinate is
+    // stepped tile by tile while exact tiles remain ahead (uniform tile loop, no reads in
+    // it), and stored at each exact tile's first in-box column.
+    const float vs = p.vs;
+    const F3 pp{p.origin[0] + (float)f.x0 * vs, p.origin[1] + (float)y * vs, p.origin[2] + (float)z * vs};
+    F3 c = f3add(mulv(f.R_wc, pp), f.t_wc);
+    int x = f.x0;
+    for (int base = t0; base <= t1; base += 16) {
+        int v[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) v[k] = row[min(base + k, t1)];
+        unsigned emask = 0u;  // bit k: tile base + k is exact
+#pragma unroll
+        for (int k = 0; k < 16; ++k)
+            if (base + k <= t1 && v[k] >= 0) emask |= 1u << k;
+        // exact tiles in later chunks keep this row stepping through the whole chunk
+        bool later = false;
+        for (int t = base + 16; t <= t1; ++t) later = later || row[t] >= 0;
+        if (emask == 0u && !later) break;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            if ((emask >> k) == 0u && !later) break;
+            const int xa = max((base + k) * kChunk, f.x0);
+            int n = xa - x;
+            for (; n >= 8; n -= 8) {
+#pragma unroll
+                for (int u = 0; u < 8; ++u) c = f3add(c, f.step);
+            }
+            for (; n > 0; --n) c = f3add(c, f.step);
+            x = xa;
+            if ((emask >> k) & 1u) is.carry[(size_t)v[k] * kSubRows + (z % kSubRows)] = make_float4(c.x, c.y, c.z, 0.0f);
+        }
+    }
+}
+
+// Brick-map bytes of one sub-tile: mask bit j = column xt + j left a non-far voxel in a
+// row of the sub-tile (near) / in its first row (first). Columns 0-16 lie in x-brick
+// bx = xt / 16, 16-31 in bx + 1, column 0 also in bx - 1; the rows lie in z-brick
+// zb / 16, and row zb = 16 q is also the last layer of z-brick q - 1; y = 16 q is also
+// in y-brick q - 1.
+__device__ __forceinline__ void markItem(unsigned char* __restrict__ bricks, int nb, int xt, int y, int zb,
+                                         unsigned near, unsigned first) {
+    const int bx = xt / kBrick, by = y / kBrick, bz = zb / kBrick;
+    const int nyb = (y % kBrick == 0 && by > 0) ? 2 : 1;
+    for (int part = 0; part < 2; ++part) {
+        const unsigned m = part == 0 ? near : ((zb % kBrick == 0) ? first : 0u);
+        const int zq = bz - part;
+        if (m == 0 || zq < 0) continue;
+        for (int yy = 0; yy < nyb; ++yy) {
+            unsigned char* row = bricks + ((size_t)zq * nb + (by - yy)) * nb;
+            if (m & 0x1ffffu) row[bx] = 1;
+            if ((m & 0xffff0000u) && bx + 1 < nb) row[bx + 1] = 1;
+            if ((m & 1u) && bx > 0) row[bx - 1] = 1;
+        }
+    }
+}
+
+template <class S, int FD, bool RC>
+__global__ __launch_bounds__(256, 8) void integrateItems(typename S::Word* __restrict__ vox, VolParams p, Intr K,
+                                                      int W, int H, const float* __restrict__ depth, float min_d,
+                                                      float max_d, FixedDiv dtr, ItemScratch is,
+                                                      unsigned long long* __restrict__ counters,
+                                                      unsigned char* __restrict__ bricks, int nb) {
+    __shared__ float lds[4][32 * kLdsStride];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const IntFrame f = is.sh->f;
+    if (!f.ok) return;
+    // Wave w serves slot w % kSlots and takes every parts-th group of its lists, the
+    // exact groups first.
+    const int wave = __builtin_amdgcn_readfirstlane((int)blockIdx.x * 4 + wv);
+    const int parts = (int)gridDim.x * 4 / kSlots;
+    const int slot = wave % kSlots, part = wave / kSlots;
+    const unsigned long long nboth = is.sh->cnt[slot];
+    const unsigned ne = (unsigned)nboth, nfr = (unsigned)(nboth >> 32);
+    const unsigned ge = (ne + kGroupItems - 1) / kGroupItems, gf = (nfr + kGroupItems - 1) / kGroupItems;
+    const unsigned* elist = is.lists + (size_t)slot * is.cap;
+    const unsigned* flist = is.lists + ((size_t)kSlots + slot) * is.cap;
+    float* L = lds[wv];
+    const size_t res = (size_t)p.res, plane = res * res;
+    const float fW = (float)W, fH = (float)H, trunc = p.trunc;
+    unsigned n_upd = 0, n_new = 0;  // per lane, summed over the wave at the end
+    // Phase-1 lanes: row r of the group (item r / kSubRows, row r % kSubRows), column half
+    // h; phase-2 lanes: column j, row parity g.
+    const int r = lane & 31, h = lane >> 5, j = lane & 31, g = lane >> 5;
+    for (unsigned grp = (unsigned)part; grp < ge + gf; grp += (unsigned)parts) {
+        const bool exact = grp < ge;  // wave-uniform
+        const unsigned first = (exact ? grp : grp - ge) * kGroupItems;
+        const unsigned nitems = min((unsigned)kGroupItems, (exact ? ne : nfr) - first);
+        const unsigned* list = exact ? elist : flist;
+        if (exact) {
+            // Phase 1: projection, depth gather and gates of the exact items' rows (the
+            // code of integrateExact's phase 1), tsdf_new (NaN = no update) into LDS.
+            const int it = r / kSubRows, row = r % kSubRows;
+            const bool act = (unsigned)it < nitems;
+            int y, zg, xt;
+            itemUnpack(act ? list[first + it] : 0u, y, zg, xt);
+            const int z = zg * kSubRows + row;
+            const bool row_ok = act && z >= f.z0 && z <= f.z1;
+            const int js = max(f.x0 - xt, 0), n = min(kChunk, f.x1 - xt + 1);
+            F3 c{0.0f, 0.0f, 0.0f};
+            if (row_ok) {
+                const float4 q = is.carry[((size_t)slot * is.cap + first + it) * kSubRows + row];
+                c = F3{q.x, q.y, q.z};
+            }
+            for (int jj = js; jj < min(n, kSub); ++jj)
+                if (h) c = f3add(c, f.step);
+#pragma unroll 1
+            for (int j0 = h * kSub; j0 < h * kSub + kSub; j0 += kGather) {
+                float cz[kGather], dj[kGather];
+                int pix[kGather];
+#pragma unroll
+                for (int jj = 0; jj < kGather; ++jj) {
+                    pix[jj] = -1;
+                    cz[jj] = c.z;
+                    if (j0 + jj >= js && j0 + jj < n) {
+                        if (row_ok && c.z > 0.0f) {
+                            const float inv_z = recipBy<RC>(c.z);
+                            const float uf = floorf(K.fx * c.x * inv_z + K.cx + 0.5f);
+                            const float vf = floorf(K.fy * c.y * inv_z + K.cy + 0.5f);
+                            if (uf >= 0.0f && uf < fW && vf >= 0.0f && vf < fH) pix[jj] = (int)vf * W + (int)uf;
+                        }
+                        c = f3add(c, f.step);
+                    }
+                }
+#pragma unroll
+                for (int jj = 0; jj < kGather; ++jj) dj[jj] = depth[pix[jj] < 0 ? 0 : pix[jj]];
+#pragma unroll
+                for (int jj = 0; jj < kGather; ++jj) {
+                    float tn = __builtin_nanf("");
+                    const float d = dj[jj];
+                    if (pix[jj] >= 0 && d >= min_d && d <= max_d) {
+                        const float sdf = d - cz[jj];
+                        if (!(sdf < -trunc)) tn = smin(1.0f, divBy<FD>(dtr, sdf));
+                    }
+                    L[r * kLdsStride + j0 + jj] = tn;
+                }
+            }
+            waveSync();
+        }
+        // Phase 2, two items at a time: lane = column, rows g, g + 2, g + 4, g + 6 of each
+        // item; every load of the pair is issued before the first fold. Voxels are read
+        // and written through a buffer resource based at the item's first voxel; a lane
+        // with nothing to fold reads at the out-of-range offset 2^31 (no access).
+        constexpr int kPairRows = kSubRows / 2;
+        const unsigned lane_off = (unsigned)(((size_t)g * plane + (size_t)j) * sizeof(typename S::Word));
+        const unsigned row_step = (unsigned)(2 * plane * sizeof(typename S::Word));
+#pragma unroll 1
+        for (unsigned i0 = 0; i0 < nitems; i0 += 2) {
+            __amdgpu_buffer_rsrc_t rs[2];
+            int iy[2], izb[2], ixt[2];
+            float tn[2 * kPairRows];
+            typename S::Word wd[2 * kPairRows];
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                const bool has = i0 + q < nitems;
+                int y, zg, xt;
+                itemUnpack(has ? __builtin_amdgcn_readfirstlane(list[first + i0 + q]) : 0u, y, zg, xt);
+                const int zb = zg * kSubRows;
+                iy[q] = y;
+                izb[q] = zb;
+                ixt[q] = xt;
+                const size_t item0 = (size_t)zb * plane + (size_t)y * res + (size_t)xt;
+                const size_t left = (res * plane - item0) * sizeof(typename S::Word);
+                rs[q] = __builtin_amdgcn_make_buffer_rsrc(vox + item0, (short)0,
+                                                          (int)(left < 0x80000000ull ? left : 0x80000000ull),
+                                                          0x00020000);
+                const int js = max(f.x0 - xt, 0), n = min(kChunk, f.x1 - xt + 1);
+                const bool col_ok = has && j >= js && j < n;
+#pragma unroll
+                for (int k = 0; k < kPairRows; ++k) {
+                    const int row = g + 2 * k, z = zb + row;
+                    const float t = exact ? L[((int)(i0 + q) * kSubRows + row) * kLdsStride + j] : 1.0f;
+                    tn[q * kPairRows + k] = (col_ok && z >= f.z0 && z <= f.z1) ? t : __builtin_nanf("");
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 2 * kPairRows; ++k)
+                wd[k] = BufWord<S>::load(rs[k / kPairRows], __builtin_isnan(tn[k]) ? 0x80000000u : lane_off,
+                                         (unsigned)(k % kPairRows) * row_step);
+            // Branch-free folds: every lane computes one, and lanes without an update
+            // store at the out-of-range offset (no access). nfb bit k: the fold left a
+            // non-far voxel.
+            unsigned nfb = 0u;
+#pragma unroll
+            for (int k = 0; k < 2 * kPairRows; ++k) {
+                const bool upd = !__builtin_isnan(tn[k]);
+                const float w_old = S::weight(wd[k]);
+                const float t_old = S::tsdf(wd[k]);
+                const float t_new = (t_old * w_old + tn[k]) / (w_old + 1.0f);
+                BufWord<S>::store(S::pack(t_new, smin(w_old + 1.0f, p.max_w)), rs[k / kPairRows],
+                                  upd ? lane_off : 0x80000000u, (unsigned)(k % kPairRows) * row_step);
+                n_upd += upd ? 1u : 0u;
+                n_new += (upd && w_old == 0.0f) ? 1u : 0u;
+                nfb |= (upd && !S::farPacked(t_new)) ? (1u << k) : 0u;  // the new weight is >= 1
+            }
+            if (bricks) {
+                constexpr unsigned kItemBits = (1u << kPairRows) - 1u;
+                const unsigned long long near0 = __ballot((nfb & kItemBits) != 0u);
+                const unsigned long long near1 = __ballot((nfb >> kPairRows) != 0u);
+                const unsigned long long first0 = __ballot(g == 0 && (nfb & 1u));
+                const unsigned long long first1 = __ballot(g == 0 && ((nfb >> kPairRows) & 1u));
+                if (lane == 0) {
+                    const unsigned c0 = (unsigned)(near0 | (near0 >> 32)), c1 = (unsigned)(near1 | (near1 >> 32));
+                    if (c0) markItem(bricks, nb, ixt[0], iy[0], izb[0], c0, (unsigned)first0);
+                    if (c1) markItem(bricks, nb, ixt[1], iy[1], izb[1], c1, (unsigned)first1);
+                }
+            }
+        }
         waveSync();
     }
     if (counters) {
@@ -875,7 +1151,8 @@ size_t depthBoundsBytes(int W, int H) {
 // The depth-bounds pyramid of the frame into `bounds` (null: no bounds, every
 // in-frustum tile is decided voxel by voxel).
 static DepthBounds boundsFor(void* bounds, const float* depth, int W, int H, float min_d, float max_d,
-                             hipStream_t st) {
+                             hipStream_t st, IntShared* sh = nullptr, const VolParams& p = VolParams{},
+                             const PoseArg& pa = PoseArg{}, Intr K = Intr{}) {
     DepthBounds bnd{};
     if (!bounds) return bnd;
     int off = 0;
@@ -887,47 +1164,57 @@ static DepthBounds boundsFor(void* bounds, const float* depth, int W, int H, flo
     }
     bnd.lvl = static_cast<const float2*>(bounds);
     hipLaunchKernelGGL(depthBoundsKernel, dim3((W + 63) / 64, (H + 63) / 64), dim3(256), 0, st, depth, W, H, min_d,
-                       max_d, static_cast<float2*>(bounds), bnd);
+                       max_d, static_cast<float2*>(bounds), bnd, sh, p, pa, K);
     return bnd;
 }
 
-// bricks: the brick map to keep (brickCount(res)^3 bytes), or null. classes: the exact
-// kernel's per-frame scratch (exactScratchBytes(res): tile-class map and row carries), or
-// null to classify and carry inside the kernel.
+// bricks: the brick map to keep (brickCount(res)^3 bytes), or null. classes: the item
+// pipeline's per-frame scratch (exactScratchBytes(res)), or null to run integrateExact,
+// which classifies and carries inside the kernel (needs no scratch besides the bounds).
 hipError_t launchIntegrate(int format, bool exact, void* vox, const VolParams& p, Intr K, int W, int H,
                            const float* depth, float min_d, float max_d, const PoseArg& pa, const FixedDiv& dtr,
                            bool recip, void* bounds, unsigned long long* counters, unsigned char* bricks,
                            unsigned char* classes, hipStream_t st) {
     const int nb = brickCount(p.res);
     if (exact) {
+        // sdf = d - c.z with d >= min_d: when min_d >= 2^-60 a nonzero sdf cannot fall below
+        // 2^-100 and the unguarded fixed-divisor form runs.
+        const bool fd = dtr.fast != 0, rc = recip, far = min_d >= 0x1p-60f;
+        const int W_ = W;
+        if (classes && bounds) {
+            const ItemScratch is = itemScratch(classes, p.res);
+            const DepthBounds bnd = boundsFor(bounds, depth, W, H, min_d, max_d, st, is.sh, p, pa, K);
+            hipLaunchKernelGGL(subtileClassKernel, dim3(classBlocks(p.res)), dim3(256), 0, st, p, K, W, H, max_d, bnd,
+                               is);
+            // every row of the volume (rows outside the frame's box exit at once)
+            const long long rows = (long long)p.res * p.res;
+            hipLaunchKernelGGL(itemCarryKernel, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0, st, p, is);
+#define AZU_INT_ITEMS(S, W, FD, RC)                                                                               \
+    hipLaunchKernelGGL((integrateItems<S, FD, RC>), dim3(kItemBlocks), dim3(256), 0, st, (W*)vox, p, K, W_, H, depth, \
+                       min_d, max_d, dtr, is, counters, bricks, nb)
+            if (format == kFmtF32) {
+                if (fd && far && rc) AZU_INT_ITEMS(StoreF32, float2, 2, true);
+                else if (fd && rc) AZU_INT_ITEMS(StoreF32, float2, 1, true);
+                else if (rc) AZU_INT_ITEMS(StoreF32, float2, 0, true);
+                else AZU_INT_ITEMS(StoreF32, float2, 0, false);
+            } else {
+                if (fd && far && rc) AZU_INT_ITEMS(StoreF16, uint32_t, 2, true);
+                else if (fd && rc) AZU_INT_ITEMS(StoreF16, uint32_t, 1, true);
+                else if (rc) AZU_INT_ITEMS(StoreF16, uint32_t, 0, true);
+                else AZU_INT_ITEMS(StoreF16, uint32_t, 0, false);
+            }
+#undef AZU_INT_ITEMS
+            return hipGetLastError();
+        }
         const DepthBounds bnd = boundsFor(bounds, depth, W, H, min_d, max_d, st);
         // One wave per (y, kTileRows-row z group, row segment) of the whole volume; waves
         // outside the frustum box exit at once (the box is only known on the device).
         const long long tasks =
             (long long)p.res * ((p.res + kTileRows - 1) / kTileRows + 1) * ((p.res + kSegCols - 1) / kSegCols + 1);
         const dim3 grid((unsigned)((tasks + 3) / 4)), block(256);
-        // sdf = d - c.z with d >= min_d: when min_d >= 2^-60 a nonzero sdf cannot fall below
-        // 2^-100 and the unguarded fixed-divisor form runs.
-        const bool fd = dtr.fast != 0, rc = recip, far = min_d >= 0x1p-60f;
-        const unsigned char* cmap = nullptr;
-        const float4* carry = nullptr;
-        if (classes && bnd.lvl) {
-            const long long tiles = (long long)p.res * tileGrid(p.res) * tileGrid(p.res);
-            hipLaunchKernelGGL(tileClassKernel, dim3((unsigned)((tiles + 255) / 256)), dim3(256), 0, st, p, K, W, H,
-                               max_d, pa, bnd, classes);
-            cmap = classes;
-        }
-        if (classes) {
-            float4* rows = reinterpret_cast<float4*>(classes + carryOffset(p.res));
-            const long long n = (long long)p.res * p.res;
-            hipLaunchKernelGGL(rowCarryKernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, K, W, H, max_d,
-                               pa, cmap, rows);
-            carry = rows;
-        }
 #define AZU_INT_EXACT(S, W, FD, RC)                                                                        \
     hipLaunchKernelGGL((integrateExact<S, FD, RC>), grid, block, 0, st, (W*)vox, p, K, W_, H, depth, min_d, \
-                       max_d, pa, dtr, bnd, counters, bricks, nb, cmap, carry)
-        const int W_ = W;
+                       max_d, pa, dtr, bnd, counters, bricks, nb)
         if (format == kFmtF32) {
             if (fd && far && rc) AZU_INT_EXACT(StoreF32, float2, 2, true);
             else if (fd && rc) AZU_INT_EXACT(StoreF32, float2, 1, true);
