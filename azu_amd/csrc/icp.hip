@@ -15,7 +15,7 @@
 //      and loss, and accumulates the 21 upper-triangle A terms, 6 b terms and the
 //      residual as Kahan-compensated FP32 partial sums, plus 6 integer counters; each
 //      wave sums its lanes' compensated sums by a fixed DPP tree, and the block adds its
-//      four wave sums in FP64 and writes one FP64 partial per slot.
+//      wave sums in FP64 and writes one FP64 partial per slot.
 // The tracker state (pose, per-level result, stop flag) lives in device memory, so all
 // launches of a frame are enqueued back to back; once a level has stopped, its remaining
 // launches return at once.
@@ -27,6 +27,8 @@ constexpr int kIcpSlotsF = 28;  // 21 A + 6 b + residual
 constexpr int kIcpSlotsI = 6;   // inliers, valid_live, valid_model, projected, dist_f, angle_f
 constexpr int kIcpSlots = kIcpSlotsF + kIcpSlotsI;
 constexpr int kIcpMaxBlocks = 256;
+constexpr int kIcpThreads = 512;              // threads per block: two waves per SIMD
+constexpr int kIcpWaves = kIcpThreads / 64;
 constexpr int kSumGroups = 7;  // cross-block sum: 7 threads per slot (7 * 34 = 238 <= 256)
 
 // Pose terms every thread of an iteration uses: the current pose (camera to world) and
@@ -656,11 +658,11 @@ __device__ __forceinline__ F3 load3(const float* __restrict__ p, int i) { return
 
 constexpr int kPxBatch = 5;   // pixels whose reads one thread issues together
 
-// Blocks an iteration over a W x H level is split into: one per 256 interior pixels,
+// Blocks an iteration over a W x H level is split into: one per kIcpThreads interior pixels,
 // at most kIcpMaxBlocks.
 __host__ __device__ int icpBlocksFor(int W, int H) {
     const long long npx = (long long)(W > 2 ? W - 2 : 0) * (H > 2 ? H - 2 : 0);
-    long long b = (npx + 255) / 256;
+    long long b = (npx + kIcpThreads - 1) / kIcpThreads;
     if (b < 1) b = 1;
     if (b > kIcpMaxBlocks) b = kIcpMaxBlocks;
     return (int)b;
@@ -690,8 +692,8 @@ __device__ __forceinline__ void loadBatch(const IcpIterArgs& a, int p0, int stri
 }
 
 struct IcpShared {
-    float wsum[4][kIcpSlotsF];  // per-wave sums
-    int cnt[4][kIcpSlotsI];
+    float wsum[kIcpWaves][kIcpSlotsF];  // per-wave sums
+    int cnt[kIcpWaves][kIcpSlotsI];
     double grp[kSumGroups][kIcpSlots];
     double sum[kIcpSlots];
     SolveLanes lanes;
@@ -702,8 +704,8 @@ struct IcpShared {
 
 // One block's share of an iteration: the pixel pass over its pixels, the block fold and
 // its FP64 partials (written by wave 0). The level is split over the launch's blocks;
-// first: the thread's first batch (loadBatch at p0 = blockIdx.x * 256 + tid, stride =
-// gridDim.x * 256), already requested.
+// first: the thread's first batch (loadBatch at p0 = blockIdx.x * kIcpThreads + tid, stride =
+// gridDim.x * kIcpThreads), already requested.
 __device__ void icpBlockPass(const IcpIterArgs& a, const PoseDerived& pd, double* __restrict__ partials,
                              IcpShared& sh, const PassBatch& first) {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -716,11 +718,11 @@ __device__ void icpBlockPass(const IcpIterArgs& a, const PoseDerived& pd, double
     int cnt[kIcpSlotsI] = {0, 0, 0, 0, 0, 0};
     const int iw = a.W - 2, ih = a.H - 2;
     const int npx = (iw > 0 && ih > 0) ? iw * ih : 0;
-    const int stride = (int)gridDim.x * 256;
+    const int stride = (int)gridDim.x * kIcpThreads;
     // Each thread visits pixels p0, p0 + stride, ... in that order, kPxBatch at a time:
     // the batch's live reads, then all of its model gathers, are issued before their use.
     PassBatch pb = first;
-    const int p_first = blockIdx.x * 256 + tid;
+    const int p_first = blockIdx.x * kIcpThreads + tid;
     for (int p0 = p_first; p0 < npx; p0 += kPxBatch * stride) {
         if (p0 != p_first) loadBatch(a, p0, stride, pb);
         int midx[kPxBatch];
@@ -738,7 +740,7 @@ __device__ void icpBlockPass(const IcpIterArgs& a, const PoseDerived& pd, double
             if (midx[b] >= 0) icpAccumulate(a, pd, pb.lv[b], mv[b], mn[b], pb.ln[b], acc, comp, cnt);
     }
 
-    // Block fold: each thread's Kahan sum (s - c), a fixed DPP tree per wave, then the four
+    // Block fold: each thread's Kahan sum (s - c), a fixed DPP tree per wave, then the
     // wave sums in FP64 in wave order. Counters are exact integers.
     float wsum[kIcpSlotsF];
 #pragma unroll
@@ -755,11 +757,15 @@ __device__ void icpBlockPass(const IcpIterArgs& a, const PoseDerived& pd, double
     if (tid < kIcpSlots) {
         double v;
         if (tid < kIcpSlotsF) {
-            v = (((double)sh.wsum[0][tid] + (double)sh.wsum[1][tid]) + (double)sh.wsum[2][tid]) +
-                (double)sh.wsum[3][tid];
+            v = (double)sh.wsum[0][tid];
+#pragma unroll
+            for (int w = 1; w < kIcpWaves; ++w) v += (double)sh.wsum[w][tid];
         } else {
             const int i = tid - kIcpSlotsF;
-            v = (double)(sh.cnt[0][i] + sh.cnt[1][i] + sh.cnt[2][i] + sh.cnt[3][i]);
+            int n = 0;
+#pragma unroll
+            for (int w = 0; w < kIcpWaves; ++w) n += sh.cnt[w][i];
+            v = (double)n;
         }
         // Block-major (coalesced both ways); the next launch reads them.
         partials[(size_t)blockIdx.x * kIcpSlots + tid] = v;
@@ -825,7 +831,7 @@ struct IcpPrev {
     int level, iter, npx, nblk;
 };
 
-__global__ __launch_bounds__(256) void icpIter(IcpIterArgs a, IcpPrev prev, IcpState* __restrict__ st,
+__global__ __launch_bounds__(kIcpThreads) void icpIter(IcpIterArgs a, IcpPrev prev, IcpState* __restrict__ st,
                                                const double* __restrict__ part_prev, double* __restrict__ part_cur,
                                                int par) {
     __shared__ IcpShared sh;
@@ -850,7 +856,7 @@ __global__ __launch_bounds__(256) void icpIter(IcpIterArgs a, IcpPrev prev, IcpS
     // Later launches of a level request it once the state shows the level still running
     // (otherwise the launch has nothing to do), in time to arrive under the solve.
     const bool level_start = level >= 0 && level != prev.level;
-    const int p_first = blockIdx.x * 256 + tid, stride = (int)gridDim.x * 256;
+    const int p_first = blockIdx.x * kIcpThreads + tid, stride = (int)gridDim.x * kIcpThreads;
     PassBatch first;
     if (level_start) loadBatch(a, p_first, stride, first);
     const int stop = st->stop_level;
@@ -859,8 +865,11 @@ __global__ __launch_bounds__(256) void icpIter(IcpIterArgs a, IcpPrev prev, IcpS
     const Rigid ref_inv = st->ref_inv;
     // The previous launch's partials (used only if it ran a pass) are requested before
     // the state arrives.
+    // (Only the threads of the sum's groups read them; the waves past them skip the reads.)
     double x[kMaxPer];
-    if (prev.level >= 0) loadPartials(part_prev, prev.nblk, x);
+#pragma unroll
+    for (int i = 0; i < kMaxPer; ++i) x[i] = 0.0;
+    if (prev.level >= 0 && tid < (kSumGroups * kIcpSlots + 63) / 64 * 64) loadPartials(part_prev, prev.nblk, x);
     const bool late = !level_start && a.level >= 0 && stop != a.level;
     bool stopped = false;
     const bool solve = pending && prev.level >= 0;
@@ -957,7 +966,7 @@ hipError_t launchIcpIter(const float* lv, const float* ln, int W, int H, const f
     double* cur = partials + (size_t)(k & 1) * half;
     const double* old = partials + (size_t)((k + 1) & 1) * half;
     const unsigned blocks = level >= 0 ? (unsigned)icpBlocksFor(W, H) : 1u;
-    hipLaunchKernelGGL(icpIter, dim3(blocks), dim3(256), 0, s, a, prev, (IcpState*)st, old, cur, k & 1);
+    hipLaunchKernelGGL(icpIter, dim3(blocks), dim3(kIcpThreads), 0, s, a, prev, (IcpState*)st, old, cur, k & 1);
     return hipGetLastError();
 }
 
