@@ -725,7 +725,9 @@ __device__ __forceinline__ void markItem(unsigned char* __restrict__ bricks, int
     }
 }
 
-template <class S, int FD, bool RC>
+// ONE: the volume spans at most 2^31 bytes, so one buffer resource over the whole volume
+// serves every item (the item's first voxel as the uniform offset).
+template <class S, int FD, bool RC, bool ONE>
 __global__ __launch_bounds__(256, 8) void integrateItems(typename S::Word* __restrict__ vox, VolParams p, Intr K,
                                                       int W, int H, const float* __restrict__ depth, float min_d,
                                                       float max_d, FixedDiv dtr, ItemScratch is,
@@ -748,6 +750,9 @@ __global__ __launch_bounds__(256, 8) void integrateItems(typename S::Word* __res
     float* L = lds[wv];
     const size_t res = (size_t)p.res, plane = res * res;
     const float fW = (float)W, fH = (float)H, trunc = p.trunc;
+    // (unused without ONE)
+    const __amdgpu_buffer_rsrc_t vrs = __builtin_amdgcn_make_buffer_rsrc(
+        vox, (short)0, ONE ? (int)(res * plane * sizeof(typename S::Word)) : 0, 0x00020000);
     unsigned n_upd = 0, n_new = 0;  // per lane, summed over the wave at the end
     // Phase-1 lanes: row r of the group (item r / kSubRows, row r % kSubRows), column half
     // h; phase-2 lanes: column j, row parity g.
@@ -817,6 +822,7 @@ __global__ __launch_bounds__(256, 8) void integrateItems(typename S::Word* __res
 #pragma unroll 1
         for (unsigned i0 = 0; i0 < nitems; i0 += 2) {
             __amdgpu_buffer_rsrc_t rs[2];
+            unsigned ibase[2];  // ONE: the item's first voxel's byte offset
             int iy[2], izb[2], ixt[2];
             float tn[2 * kPairRows];
             typename S::Word wd[2 * kPairRows];
@@ -830,10 +836,16 @@ __global__ __launch_bounds__(256, 8) void integrateItems(typename S::Word* __res
                 izb[q] = zb;
                 ixt[q] = xt;
                 const size_t item0 = (size_t)zb * plane + (size_t)y * res + (size_t)xt;
-                const size_t left = (res * plane - item0) * sizeof(typename S::Word);
-                rs[q] = __builtin_amdgcn_make_buffer_rsrc(vox + item0, (short)0,
-                                                          (int)(left < 0x80000000ull ? left : 0x80000000ull),
-                                                          0x00020000);
+                if (ONE) {
+                    rs[q] = vrs;
+                    ibase[q] = (unsigned)(item0 * sizeof(typename S::Word));
+                } else {
+                    const size_t left = (res * plane - item0) * sizeof(typename S::Word);
+                    rs[q] = __builtin_amdgcn_make_buffer_rsrc(vox + item0, (short)0,
+                                                              (int)(left < 0x80000000ull ? left : 0x80000000ull),
+                                                              0x00020000);
+                    ibase[q] = 0u;
+                }
                 const int js = max(f.x0 - xt, 0), n = min(kChunk, f.x1 - xt + 1);
                 const bool col_ok = has && j >= js && j < n;
 #pragma unroll
@@ -846,7 +858,7 @@ __global__ __launch_bounds__(256, 8) void integrateItems(typename S::Word* __res
 #pragma unroll
             for (int k = 0; k < 2 * kPairRows; ++k)
                 wd[k] = BufWord<S>::load(rs[k / kPairRows], __builtin_isnan(tn[k]) ? 0x80000000u : lane_off,
-                                         (unsigned)(k % kPairRows) * row_step);
+                                         ibase[k / kPairRows] + (unsigned)(k % kPairRows) * row_step);
             // Branch-free folds: every lane computes one, and lanes without an update
             // store at the out-of-range offset (no access). nfb bit k: the fold left a
             // non-far voxel.
@@ -858,7 +870,7 @@ __global__ __launch_bounds__(256, 8) void integrateItems(typename S::Word* __res
                 const float t_old = S::tsdf(wd[k]);
                 const float t_new = (t_old * w_old + tn[k]) / (w_old + 1.0f);
                 BufWord<S>::store(S::pack(t_new, smin(w_old + 1.0f, p.max_w)), rs[k / kPairRows],
-                                  upd ? lane_off : 0x80000000u, (unsigned)(k % kPairRows) * row_step);
+                                  upd ? lane_off : 0x80000000u, ibase[k / kPairRows] + (unsigned)(k % kPairRows) * row_step);
                 n_upd += upd ? 1u : 0u;
                 n_new += (upd && w_old == 0.0f) ? 1u : 0u;
                 nfb |= (upd && !S::farPacked(t_new)) ? (1u << k) : 0u;  // the new weight is >= 1
@@ -1189,9 +1201,16 @@ hipError_t launchIntegrate(int format, bool exact, void* vox, const VolParams& p
             // every row of the volume (rows outside the frame's box exit at once)
             const long long rows = (long long)p.res * p.res;
             hipLaunchKernelGGL(itemCarryKernel, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0, st, p, is);
-#define AZU_INT_ITEMS(S, W, FD, RC)                                                                               \
-    hipLaunchKernelGGL((integrateItems<S, FD, RC>), dim3(kItemBlocks), dim3(256), 0, st, (W*)vox, p, K, W_, H, depth, \
-                       min_d, max_d, dtr, is, counters, bricks, nb)
+            const bool one = (size_t)p.res * p.res * p.res * (format == kFmtF32 ? 8 : 4) <= ((size_t)1 << 31);
+#define AZU_INT_ITEMS(S, W, FD, RC)                                                                                   \
+    do {                                                                                                              \
+        if (one)                                                                                                      \
+            hipLaunchKernelGGL((integrateItems<S, FD, RC, true>), dim3(kItemBlocks), dim3(256), 0, st, (W*)vox, p, K, \
+                               W_, H, depth, min_d, max_d, dtr, is, counters, bricks, nb);                            \
+        else                                                                                                          \
+            hipLaunchKernelGGL((integrateItems<S, FD, RC, false>), dim3(kItemBlocks), dim3(256), 0, st, (W*)vox, p,   \
+                               K, W_, H, depth, min_d, max_d, dtr, is, counters, bricks, nb);                         \
+    } while (0)
             if (format == kFmtF32) {
                 if (fd && far && rc) AZU_INT_ITEMS(StoreF32, float2, 2, true);
                 else if (fd && rc) AZU_INT_ITEMS(StoreF32, float2, 1, true);
