@@ -206,19 +206,29 @@ __device__ __forceinline__ bool sampleVol(const Vol<S>& V, const VolBuf& R, F3 p
     return sampleEval<S, E>(c, f);
 }
 
-// computeNormal (TSDFVolume.cpp:478-490)
+// computeNormal (TSDFVolume.cpp:478-490). The six samples' reads are all issued before
+// the first is evaluated (three at a time for the 16-byte FP32 pairs), and the normal is
+// withdrawn when any of them is invalid: the reference stops at the first invalid one, and
+// either way there is no normal.
 template <class S, bool E, int FD, bool B>
 __device__ __forceinline__ bool normalVol(const Vol<S>& V, const VolBuf& R, F3 p, F3& n) {
     const float h = V.vs;
-    float s0, s1, s2, s3, s4, s5;
-    if (!sampleVol<S, E, FD, B>(V, R, F3{p.x + h, p.y + 0.0f, p.z + 0.0f}, s0) ||
-        !sampleVol<S, E, FD, B>(V, R, F3{p.x - h, p.y - 0.0f, p.z - 0.0f}, s1) ||
-        !sampleVol<S, E, FD, B>(V, R, F3{p.x + 0.0f, p.y + h, p.z + 0.0f}, s2) ||
-        !sampleVol<S, E, FD, B>(V, R, F3{p.x - 0.0f, p.y - h, p.z - 0.0f}, s3) ||
-        !sampleVol<S, E, FD, B>(V, R, F3{p.x + 0.0f, p.y + 0.0f, p.z + h}, s4) ||
-        !sampleVol<S, E, FD, B>(V, R, F3{p.x - 0.0f, p.y - 0.0f, p.z - h}, s5))
-        return false;
-    const F3 g{s0 - s1, s2 - s3, s4 - s5};
+    const F3 q[6] = {F3{p.x + h, p.y + 0.0f, p.z + 0.0f}, F3{p.x - h, p.y - 0.0f, p.z - 0.0f},
+                     F3{p.x + 0.0f, p.y + h, p.z + 0.0f}, F3{p.x - 0.0f, p.y - h, p.z - 0.0f},
+                     F3{p.x + 0.0f, p.y + 0.0f, p.z + h}, F3{p.x - 0.0f, p.y - 0.0f, p.z - h}};
+    constexpr int kBatch = S::kBytes == 8 ? 3 : 6;
+    float s[6];
+    bool ok = true;
+#pragma unroll
+    for (int b0 = 0; b0 < 6; b0 += kBatch) {
+        Cell<S> c[kBatch];
+#pragma unroll
+        for (int i = 0; i < kBatch; ++i) c[i] = sampleLoad<S, E, FD, B>(V, R, q[b0 + i]);
+#pragma unroll
+        for (int i = 0; i < kBatch; ++i) ok = sampleEval<S, E>(c[i], s[b0 + i]) && ok;
+    }
+    if (!ok) return false;
+    const F3 g{s[0] - s[1], s[2] - s[3], s[4] - s[5]};
     const float len = f3norm(g);
     if (!(len > 1e-6f) || !isfinite(len)) return false;
     n = f3div(g, len);
