@@ -119,8 +119,9 @@ __device__ __forceinline__ Cell<S> sampleLoad(const Vol<S>& V, const VolBuf& R, 
         c.q3 = IO::loadB(R.rs, off, R.sy + R.sz);
         return c;
     }
+    // The in-plane index in 32 bits (below res^2), the plane offset in 64.
     const size_t sy = (size_t)V.res, sz = (size_t)V.res * (size_t)V.res;
-    const size_t b = (size_t)z0 * sz + (size_t)y0 * sy + (size_t)x0;
+    const size_t b = (size_t)(unsigned)z0 * sz + (size_t)((unsigned)y0 * (unsigned)V.res + (unsigned)x0);
     c.q0 = IO::load(V.vox + b);
     c.q1 = IO::load(V.vox + b + sy);
     c.q2 = IO::load(V.vox + b + sz);
@@ -165,8 +166,9 @@ __device__ __forceinline__ bool sampleEvalValue(const Cell<S>& c, float& f) {
 // replaced by 1 where the sample is invalid (outside, or a corner unobserved).
 template <class S, bool E>
 __device__ __forceinline__ bool sampleEvalFlat(const Cell<S>& c, float& f) {
+    // (bitwise: both operands are evaluated, no branch)
     const bool valid =
-        c.inside && S::observed8(c.q0.a, c.q0.b, c.q1.a, c.q1.b, c.q2.a, c.q2.b, c.q3.a, c.q3.b);
+        c.inside & S::observed8(c.q0.a, c.q0.b, c.q1.a, c.q1.b, c.q2.a, c.q2.b, c.q3.a, c.q3.b);
     Cell<S> d = c;
     d.inside = true;
     float v = 1.0f;
@@ -350,29 +352,34 @@ __global__ __launch_bounds__(256) void raycastKernel(Vol<S> V, VolParams p, Pose
             prev_valid = sampleVol<S, E, FD, B>(V, R, f3add(cam, f3mul(dir, t_prev)), f_prev);
             go = !(prev_valid && (!isfinite(f_prev) || f_prev < 0.0f));
         }
-        // Once t_prev >= t_end every later sample is far: no hit can follow.
-        while (go && t_prev < t_far && t_prev < t_end) {
+        // Once t_prev >= t_end every later sample is far: no hit can follow. Each step
+        // carries its outcomes as flags and the loop has one exit test; the hit step's
+        // bracket (t_a, f_a) -> (t_prev, f_prev) is refined after the loop.
+        bool more = go && t_prev < t_far && t_prev < t_end;
+        float t_a = 0.0f, f_a = 0.0f;
+        while (more) {
             const float step =
                 (!prev_valid || f_prev >= 0.999f) ? far_step : smax(min_step, 0.8f * f_prev * trunc);
             const float t = smin(t_prev + step, t_far);
-            // One step with its outcomes as flags (one exit branch): a non-finite value
-            // stops the march; with both samples valid, + -> - is the hit (secant step)
-            // and - -> + stops; a valid negative first sample after an invalid one stops.
+            // A non-finite value stops the march; with both samples valid, + -> - is the
+            // hit and - -> + stops; a valid negative first sample after an invalid one
+            // stops. A step that does not stop leaves t < t_far.
             float f;
             const bool valid = sampleEvalFlat<S, E>(sampleLoad<S, E, FD, B>(V, R, f3add(cam, f3mul(dir, t))), f);
-            const bool fin = isfinite(f), both = valid && prev_valid;
-            const bool hit = both && fin && f_prev > 0.0f && f <= 0.0f;
-            const bool stop = hit || (valid && !fin) || (both && f_prev < 0.0f && f >= 0.0f) ||
-                              (valid && !prev_valid && f < 0.0f) || t >= t_far;
-            if (hit) {
-                t_hit = t_prev + (t - t_prev) * (f_prev / (f_prev - f));
-                found = true;
-            }
+            // (bitwise: every flag is evaluated, no branch)
+            const bool fin = isfinite(f), both = valid & prev_valid;
+            const bool hit = both & fin & (f_prev > 0.0f) & (f <= 0.0f);
+            const bool stop = hit | (valid & !fin) | (both & (f_prev < 0.0f) & (f >= 0.0f)) |
+                              (valid & !prev_valid & (f < 0.0f)) | (t >= t_far);
+            t_a = t_prev;
+            f_a = f_prev;
             t_prev = t;
             f_prev = f;
             prev_valid = valid;
-            if (stop) break;
+            found = hit;
+            more = !stop & (t < t_end);
         }
+        if (found) t_hit = t_a + (t_prev - t_a) * (f_a / (f_a - f_prev));
         if (found) {
             hit = f3add(cam, f3mul(dir, t_hit));
             found = normalVol<S, E, FD, B>(V, R, hit, nrm) && f3finite(hit);
