@@ -129,12 +129,12 @@ struct StoreF16 {
 
 // ---------------------------------------------------------------------------
 // Division by a divisor fixed for a whole launch (the voxel size, the truncation).
-// divFixed computes q = x r (r = 1/b rounded to float) and corrects it twice with the
-// exact residual x - q b (Markstein: once q is within an ulp, a correction with the
-// correctly rounded reciprocal gives the correctly rounded quotient; the argument
-// fails where the quotient underflows). It replaces the IEEE quotient x / b only after
-// verifyFixedDivKernel has compared the two on the device for every float with
-// 2^-100 <= |x| < 2^32 and found them equal. Smaller |x| takes the IEEE division; a
+// divFixed computes q = x r (r = 1/b rounded to float) and corrects it once with the
+// exact residual x - q b (Markstein's form; whether it gives the correctly rounded
+// quotient for every x depends on b, and it fails where the quotient underflows). It
+// replaces the IEEE quotient x / b only after verifyKernel has compared the two on the
+// device for every float with 2^-100 <= |x| < 2^32 and found them equal (for the voxel
+// sizes and truncations 1 - 12 mm it holds). Smaller |x| takes the IEEE division; a
 // larger |x| only arises for a sample or distance that both forms reject or clamp (far
 // outside the volume, or sdf / trunc > 1 folded as 1).
 // ---------------------------------------------------------------------------
@@ -145,8 +145,7 @@ struct FixedDiv {
 
 __device__ __forceinline__ float divFixed(const FixedDiv& d, float x) {
     const float q0 = x * d.r;
-    const float q1 = fmaf(fmaf(-q0, d.b, x), d.r, q0);
-    return fmaf(fmaf(-q1, d.b, x), d.r, q1);
+    return fmaf(fmaf(-q0, d.b, x), d.r, q0);
 }
 
 // FD = 0: IEEE division. FD = 1: divFixed where verified (|x| >= 2^-100), IEEE below.
