@@ -109,8 +109,11 @@ __device__ __forceinline__ Cell<S> sampleLoad(const Vol<S>& V, const VolBuf& R, 
     c.tz = gz - fz0;
     using IO = PairIO<S>;
     if (B) {
+        // 24-bit multiplies: the volume spans < 2^31 bytes, so res < 2^11.
         const unsigned off =
-            c.inside ? (((unsigned)z0 * (unsigned)V.res + (unsigned)y0) * (unsigned)V.res + (unsigned)x0) *
+            c.inside ? ((unsigned)__umul24((unsigned)__umul24((unsigned)z0, (unsigned)V.res) + (unsigned)y0,
+                                           (unsigned)V.res) +
+                        (unsigned)x0) *
                            (unsigned)S::kBytes
                      : 0x80000000u;
         c.q0 = IO::loadB(R.rs, off, 0u);
@@ -119,9 +122,12 @@ __device__ __forceinline__ Cell<S> sampleLoad(const Vol<S>& V, const VolBuf& R, 
         c.q3 = IO::loadB(R.rs, off, R.sy + R.sz);
         return c;
     }
-    // The in-plane index in 32 bits (below res^2), the plane offset in 64.
-    const size_t sy = (size_t)V.res, sz = (size_t)V.res * (size_t)V.res;
-    const size_t b = (size_t)(unsigned)z0 * sz + (size_t)((unsigned)y0 * (unsigned)V.res + (unsigned)x0);
+    // The in-plane index by a 24-bit multiply (y0, x0 < res < 2^16), the plane offset as one
+    // 32 x 32 -> 64-bit product (res^2 < 2^32).
+    const unsigned plane = (unsigned)V.res * (unsigned)V.res;
+    const size_t sy = (size_t)V.res, sz = plane;
+    const unsigned in_plane = (unsigned)__umul24((unsigned)y0, (unsigned)V.res) + (unsigned)x0;
+    const size_t b = (size_t)(unsigned)z0 * plane + (size_t)in_plane;
     c.q0 = IO::load(V.vox + b);
     c.q1 = IO::load(V.vox + b + sy);
     c.q2 = IO::load(V.vox + b + sz);
