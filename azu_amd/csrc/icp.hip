@@ -127,29 +127,6 @@ __device__ __forceinline__ int waveSumInt(int v) {
 // ---------------------------------------------------------------------------
 // FP64 small dense algebra for the single-lane solve.
 // ---------------------------------------------------------------------------
-// Lower Cholesky factor of a symmetric 6x6 and the reciprocals of its diagonal; false
-// unless every pivot is positive (the matrix is then positive definite).
-__device__ bool cholesky6(const double* A, double* L, double* rdiag) {
-    for (int j = 0; j < 6; ++j) {
-        double s = A[j * 6 + j];
-        for (int k = 0; k < j; ++k) s -= L[j * 6 + k] * L[j * 6 + k];
-        if (!(s > 0.0)) return false;
-        // 1 / sqrt(s) from the hardware estimate and one Newton step (relative error
-        // ~1e-16, far below the float step this solve produces), then sqrt(s) = s / sqrt(s).
-        const double r0 = __builtin_amdgcn_rsq(s);
-        const double inv = r0 * (1.5 - (0.5 * s) * (r0 * r0));
-        const double d = s * inv;
-        L[j * 6 + j] = d;
-        rdiag[j] = inv;
-        for (int i = j + 1; i < 6; ++i) {
-            double t = A[i * 6 + j];
-            for (int k = 0; k < j; ++k) t -= L[i * 6 + k] * L[j * 6 + k];
-            L[i * 6 + j] = t * inv;
-        }
-    }
-    return true;
-}
-
 // Cyclic Jacobi eigenvalues (Golub & Van Loan 8.5) of a symmetric 6x6; returns min, max.
 __device__ void jacobiMinMax(const double* Ain, double& mn, double& mx) {
     double a[36];
@@ -194,10 +171,10 @@ __device__ void jacobiMinMax(const double* Ain, double& mn, double& mx) {
 // (Gershgorin) and s = max(1e-4, g / 1e7), a successful factorisation of A - s I
 // proves min_eig > s, hence neither escalation condition.
 //
-// The solve runs three candidate factorisations at once, one per lane of the last
-// block's first wave (the same code on different data, so together they cost one):
-// lane 0 the certificate A - s I, lane 1 A + base I, lane 2 A + 1.0 I; lanes 1 and 2
-// also solve for their step. Lane 0 then picks the candidate the damping rule selects.
+// The solve runs three candidate factorisations at once, one per wave of the block
+// (solveWave): wave 0 the certificate A - s I, wave 1 A + base I, wave 2 A + 1.0 I;
+// waves 1 and 2 also solve for their step. One lane then picks the candidate the damping
+// rule selects (icpSolve).
 struct SolveLanes {
     int ok[3];
     int finite;
@@ -215,40 +192,105 @@ __device__ void buildSystem(const double* sums, double* A, double* b) {
     for (int i = 0; i < 6; ++i) b[i] = sums[21 + i];
 }
 
-__device__ void solveLane(const double* sums, int level, int lane, SolveLanes* out) {
+// Slot of A[i][j] among the 21 upper-triangle sums (row-major upper triangle).
+__device__ __forceinline__ int symSlot(int i, int j) {
+    const int r = i < j ? i : j, c = i < j ? j : i;
+    return r * 6 - (r * (r - 1)) / 2 + (c - r);
+}
+
+__device__ __forceinline__ double readlaneD(double v, int l) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Candidate `cand` (shift, Cholesky factor L L^T of the shifted A, and for cand > 0 the
+// step from L y = b, L^T x = y) by one wave, lane i (< 6) holding row i of A and of L:
+// column j's pivot s = A[j][j] - sum_k L[j][k]^2 is formed on lane j and the entries below
+// it, A[i][j] - sum_k L[i][k] L[j][k] (k ascending), on lanes i > j with row j's entries
+// broadcast from lane j; the pivot's 1 / sqrt(s) comes from the hardware estimate and one
+// Newton step (relative error ~1e-16, far below the float step this solve produces), and
+// sqrt(s) = s / sqrt(s). The forward substitution broadcasts y[j] as it is formed; the
+// backward one (x[i] = (y[i] - sum_{j > i} L[j][i] x[j]) / L[i][i], j ascending) runs on
+// broadcast values in every lane. Every element takes the same operations in the same
+// order as a one-lane evaluation of those formulas (so the results are bit-identical to
+// it), and a pivot s <= 0 leaves ok false.
+__device__ void solveWave(const double* sums, int level, int cand, int lane, SolveLanes* out) {
     const double kMin = (double)1e-4f, kCond = (double)1e7f;
-    double A[36], b[6];
-    buildSystem(sums, A, b);
-    double g = 0.0;
-    bool fin = true;
-    for (int i = 0; i < 6; ++i) {
-        double r = A[i * 6 + i];
-        for (int j = 0; j < 6; ++j)
-            if (j != i) r += fabs(A[i * 6 + j]);
-        g = fmax(g, r);
-        for (int j = 0; j < 6; ++j) fin = fin && isfinite(A[i * 6 + j]);
+    const int i = lane < 6 ? lane : 5;  // lanes past 5 repeat row 5 (unused)
+    double a[6];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) a[j] = sums[symSlot(i, j)];
+    double shift;
+    if (cand == 0) {
+        // Gershgorin bound, rows in order: r_i = A[i][i] + sum_{j != i} |A[i][j]|.
+        double diag = a[0];
+#pragma unroll
+        for (int j = 1; j < 6; ++j) diag = i == j ? a[j] : diag;
+        double r = diag;
+#pragma unroll
+        for (int j = 0; j < 6; ++j) r = j != i ? r + fabs(a[j]) : r;
+        double g = 0.0;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) g = fmax(g, readlaneD(r, k));
+        shift = -(fmax(kMin, g / kCond) * (1.0 + 1e-12));
+        // Every A entry finite (the 21 sums).
+        const unsigned long long fin = __ballot(lane < 21 && isfinite(sums[lane < 21 ? lane : 0]));
+        if (lane == 0) out->finite = fin == 0x1FFFFFull ? 1 : 0;
+    } else {
+        shift = cand == 1 ? (level == 0 ? (double)0.01f : (double)0.1f) : 1.0;
     }
-    const double shift = lane == 0 ? -(fmax(kMin, g / kCond) * (1.0 + 1e-12))
-                                   : (lane == 1 ? (level == 0 ? (double)0.01f : (double)0.1f) : 1.0);
-    for (int i = 0; i < 6; ++i) A[i * 6 + i] += shift;
-    double L[36], r[6];
-    const bool ok = cholesky6(A, L, r);
-    if (ok && lane > 0) {
-        double y[6], x[6];
-        for (int i = 0; i < 6; ++i) {
-            double t = b[i];
-            for (int j = 0; j < i; ++j) t -= L[i * 6 + j] * y[j];
-            y[i] = t * r[i];
-        }
-        for (int i = 5; i >= 0; --i) {
-            double t = y[i];
-            for (int j = i + 1; j < 6; ++j) t -= L[j * 6 + i] * x[j];
-            x[i] = t * r[i];
-        }
-        for (int i = 0; i < 6; ++i) out->x[lane][i] = x[i];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) a[j] = i == j ? a[j] + shift : a[j];
+    // Factor: column j's pivot s on lane j, the entries below it on lanes i > j.
+    double L[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, rd[6];
+    bool ok = true;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+        double t = a[j];
+#pragma unroll
+        for (int k = 0; k < j; ++k) t -= L[k] * readlaneD(L[k], j);
+        const double sj = readlaneD(t, j);
+        ok = ok && sj > 0.0;
+        const double r0 = __builtin_amdgcn_rsq(sj);
+        const double inv = r0 * (1.5 - (0.5 * sj) * (r0 * r0));
+        const double d = sj * inv;
+        L[j] = i == j ? d : (i > j ? t * inv : 0.0);
+        rd[j] = inv;
     }
-    out->ok[lane] = ok ? 1 : 0;
-    if (lane == 0) out->finite = fin ? 1 : 0;
+    if (cand > 0 && ok) {
+        double t = sums[21 + i], y[6], x[6];
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+            y[j] = readlaneD(t, j) * rd[j];
+            t = i > j ? t - L[j] * y[j] : t;
+        }
+#pragma unroll
+        for (int ii = 5; ii >= 0; --ii) {
+            double u = y[ii];
+#pragma unroll
+            for (int j = ii + 1; j < 6; ++j) u -= readlaneD(L[ii], j) * x[j];
+            x[ii] = u * rd[ii];
+        }
+        if (lane < 6) {
+            double v = x[0];
+#pragma unroll
+            for (int k = 1; k < 6; ++k) v = lane == k ? x[k] : v;
+            out->x[cand][lane] = v;
+        }
+    }
+    if (lane == 0) out->ok[cand] = ok ? 1 : 0;
+}
+
+// Block 0's record of the solved system in the state (information = A in float, zero at
+// a level's first iteration when it has fewer than 10 inliers; the gate counters), one
+// entry per lane of one wave.
+__device__ void recordSystem(IcpState* st, const double* sums, int iter, int lane) {
+    const bool enough = (int)sums[kIcpSlotsF] >= 10;
+    if (lane < 36 && (enough || iter == 0))
+        st->information[lane] = enough ? (float)sums[symSlot(lane / 6, lane % 6)] : 0.0f;
+    if (lane >= 36 && lane < 41) st->counters[lane - 36] = (int)sums[kIcpSlotsF + 1 + (lane - 36)];
 }
 
 // Nearest rotation by one-sided Jacobi SVD in FP64, U V^T with U's smallest-singular-
@@ -414,7 +456,7 @@ __device__ PoseDerived derivePose(const float* pose, const Rigid& rinv, int iter
 }
 
 // One solve + pose update (trackLevel's loop body, ICPTracker.cpp:79-130), lane 0 of
-// wave 0 once solveLane has run on lanes 0-2; every block of a launch runs it on the same
+// wave 0 once solveWave has run on waves 0-2; every block of a launch runs it on the same
 // sums and gets the same result, and the writer (block 0) also records it in the state.
 // It reads no global memory (ref_inv: the state's, read at the launch's start).
 // sums: 28 FP64 slots (21 A upper, 6 b, residual) + 6 counters. out: the pose terms after
@@ -422,31 +464,26 @@ __device__ PoseDerived derivePose(const float* pose, const Rigid& rinv, int iter
 __device__ bool icpSolve(IcpState* st, bool writer, const double* sums, const SolveLanes* sl, const PoseDerived& pd,
                          const Rigid& ref_inv, int level, int iter, PoseDerived& out) {
     out = pd;
+    // (The information matrix and the gate counters are recorded by recordSystem.)
     if (writer && iter == 0) {
         st->error = 0.0f;
         st->inliers = 0;
         st->converged = 0;
         st->final_step = __builtin_inff();
-        for (int i = 0; i < 36; ++i) st->information[i] = 0.0f;
     }
     const int inliers = (int)sums[kIcpSlotsF + 0];
-    if (writer)
-        for (int i = 0; i < 5; ++i) st->counters[i] = (int)sums[kIcpSlotsF + 1 + i];
     if (inliers < 10) {  // covers buildLinearSystem's "no inlier" failure too
         if (writer) st->stop_level = level;
         return true;
     }
-    double A[36], b[6];
-    buildSystem(sums, A, b);
     const double residual = sums[27];
-    if (writer)
-        for (int i = 0; i < 36; ++i) st->information[i] = (float)A[i];
     int pick = 1;  // the level's base lambda
     if (!sl->finite) {
         pick = 2;
     } else if (!sl->ok[0]) {
         const double kMin = (double)1e-4f, kCond = (double)1e7f;
-        double mn, mx;
+        double A[36], b[6], mn, mx;
+        buildSystem(sums, A, b);
         jacobiMinMax(A, mn, mx);
         if (!isfinite(mn) || !isfinite(mx) || mn < kMin || mx > kCond * mn) pick = 2;
     }
@@ -807,17 +844,19 @@ __device__ bool blockSolve(IcpState* __restrict__ st, double gsum, IcpShared& sh
         sh.sum[tid] = acc;
     }
     __syncthreads();
-    if (tid < 64) {
-        const unsigned long long t_sum = __builtin_amdgcn_s_memrealtime();
-        if (tid < 3 && (int)sh.sum[kIcpSlotsF] >= 10) solveLane(sh.sum, level, tid, &sh.lanes);
-        waveSync();
-        if (tid == 0) {
-            PoseDerived out;
-            sh.stop = icpSolve(st, writer, sh.sum, &sh.lanes, pd, ref_inv, level, iter, out) ? 1 : 0;
-            sh.pd = out;
-            ticks[0] = t_sum - t0;
-            ticks[1] = __builtin_amdgcn_s_memrealtime() - t_sum;
-        }
+    // The three candidate factorisations on waves 0-2 at once, block 0's record of the
+    // system on wave 3, then the rest of the solve on one lane.
+    const unsigned long long t_sum = __builtin_amdgcn_s_memrealtime();
+    const int wv = tid >> 6, lane = tid & 63;
+    if (wv < 3 && (int)sh.sum[kIcpSlotsF] >= 10) solveWave(sh.sum, level, wv, lane, &sh.lanes);
+    if (wv == 3 && writer) recordSystem(st, sh.sum, iter, lane);
+    __syncthreads();
+    if (tid == 0) {
+        PoseDerived out;
+        sh.stop = icpSolve(st, writer, sh.sum, &sh.lanes, pd, ref_inv, level, iter, out) ? 1 : 0;
+        sh.pd = out;
+        ticks[0] = t_sum - t0;
+        ticks[1] = __builtin_amdgcn_s_memrealtime() - t_sum;
     }
     __syncthreads();
     return sh.stop != 0;
