@@ -437,54 +437,14 @@ void launchRay(const void* vox, const VolParams& p, const FixedDiv& dvs, const R
                            makeVol<S>(vox, p, dvs), p, pa, K, W, H, out, skip);
 }
 
-// dvs: division by the voxel size, verified exact (dvs.fast) or IEEE. The divided
-// values are sample coordinates minus the volume origin; when no origin component is
-// below 2^-60 in magnitude they cannot land in (0, 2^-100) and the unguarded form runs.
-// The per-pixel bounds of RaySkip. One wave per brick: a marked brick's span box,
-// widened by e voxels (e covers the rounding of the march's grid coordinates, which stays
-// below 4 2^-24 M voxels for a bound M on the magnitudes involved), is projected through
-// the inverse camera rotation. Only its part in front of the plane z = min_depth / 2 can
-// hold a sample (every sample lies at camera depth >= min_depth), so the corners there
-// and the box edges' crossings of the plane are projected; the perspective image of that
-// convex part is their hull, and the pixel rectangle around it, widened by one pixel,
-// takes the minimum of the box's distance from the camera (less 1e-5 of it) and the
-// maximum of its farthest point's distance (plus 1e-5 of it) in every 2 x 2 pixel cell
-// it touches, one cell per lane. A sample at parameter t is the point cam + dir t, at
-// distance t |dir|; a sample that is not far lies in some marked brick's widened box, in
-// front of the plane, so its pixel is in that brick's rectangle and its distance between
-// the two bounds.
-// ok = 0 (take every sample) unless the pose is finite and invertible and min_depth > 0.
-__global__ __launch_bounds__(256) void brickSplat(const unsigned char* __restrict__ bricks, int nb, VolParams p,
-                                                  PoseArg pa, Intr K, int W, int H, unsigned* __restrict__ dsafe,
-                                                  int* __restrict__ ok) {
-    const Rigid P = rigidFrom(pa);
-    const M33& R = P.R;
-    M33 A;  // adjugate of R
-    A.m[0][0] = R.m[1][1] * R.m[2][2] - R.m[1][2] * R.m[2][1];
-    A.m[0][1] = R.m[0][2] * R.m[2][1] - R.m[0][1] * R.m[2][2];
-    A.m[0][2] = R.m[0][1] * R.m[1][2] - R.m[0][2] * R.m[1][1];
-    A.m[1][0] = R.m[1][2] * R.m[2][0] - R.m[1][0] * R.m[2][2];
-    A.m[1][1] = R.m[0][0] * R.m[2][2] - R.m[0][2] * R.m[2][0];
-    A.m[1][2] = R.m[0][2] * R.m[1][0] - R.m[0][0] * R.m[1][2];
-    A.m[2][0] = R.m[1][0] * R.m[2][1] - R.m[1][1] * R.m[2][0];
-    A.m[2][1] = R.m[0][1] * R.m[2][0] - R.m[0][0] * R.m[2][1];
-    A.m[2][2] = R.m[0][0] * R.m[1][1] - R.m[0][1] * R.m[1][0];
-    const float det = R.m[0][0] * A.m[0][0] + R.m[0][1] * A.m[1][0] + R.m[0][2] * A.m[2][0];
-    bool valid = isfinite(det) && fabsf(det) > 1e-3f && p.min_depth > 0.0f && f3finite(P.t);
-    M33 Ri;
-#pragma unroll
-    for (int r = 0; r < 3; ++r)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            Ri.m[r][c] = A.m[r][c] / det;
-            valid = valid && isfinite(Ri.m[r][c]);
-        }
-    if (blockIdx.x == 0 && threadIdx.x == 0) *ok = valid ? 1 : 0;
-    if (!valid) return;
+constexpr int kSplatPer = 16;  // bricks per wave in brickSplat
 
-    const int lane = threadIdx.x & 63;
-    const size_t bi = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (bi >= (size_t)nb * nb * nb || bricks[bi] == 0) return;  // whole wave
+// One marked brick's share of brickSplat: its distance bounds (float bit patterns) and the
+// rectangle of 2 x 2 pixel cells (first cell, width, cell count; area 0: none) its widened
+// span box can reach.
+__device__ void splatRect(const Rigid& P, const M33& Ri, const VolParams& p, Intr K, int W, int H, int nb, size_t bi,
+                          int& cx0_out, int& cy0_out, int& w_out, int& area_out, unsigned& dlow_out,
+                          unsigned& dhigh_out) {
     const int b[3] = {(int)(bi % nb), (int)((bi / nb) % nb), (int)(bi / ((size_t)nb * nb))};
     const F3 cam = P.t;
     const float cam_a[3] = {cam.x, cam.y, cam.z};
@@ -534,18 +494,87 @@ __global__ __launch_bounds__(256) void brickSplat(const unsigned char* __restric
             vmax = fmaxf(vmax, v);
         }
     }
+    area_out = 0;
     if (!(umin <= umax)) return;  // nothing in front of the plane
     const int x0 = (int)fmaxf(floorf(fmaxf(umin, -4.0f)) - 1.0f, 0.0f);
     const int x1 = (int)fminf(ceilf(fminf(umax, (float)W + 4.0f)) + 1.0f, (float)(W - 1));
     const int y0 = (int)fmaxf(floorf(fmaxf(vmin, -4.0f)) - 1.0f, 0.0f);
     const int y1 = (int)fminf(ceilf(fminf(vmax, (float)H + 4.0f)) + 1.0f, (float)(H - 1));
     if (x0 > x1 || y0 > y1) return;
-    const int cw = (W + 1) >> 1, cx0 = x0 >> 1, cy0 = y0 >> 1;
-    const int w = (x1 >> 1) - cx0 + 1, area = w * ((y1 >> 1) - cy0 + 1);
-    for (int k = lane; k < area; k += 64) {
-        unsigned* c = dsafe + 2 * ((size_t)(cy0 + k / w) * cw + (cx0 + k % w));
-        atomicMin(c, dlow);
-        atomicMax(c + 1, dhigh);
+    cx0_out = x0 >> 1;
+    cy0_out = y0 >> 1;
+    w_out = (x1 >> 1) - cx0_out + 1;
+    area_out = w_out * ((y1 >> 1) - cy0_out + 1);
+    dlow_out = dlow;
+    dhigh_out = dhigh;
+}
+
+// dvs: division by the voxel size, verified exact (dvs.fast) or IEEE. The divided
+// values are sample coordinates minus the volume origin; when no origin component is
+// below 2^-60 in magnitude they cannot land in (0, 2^-100) and the unguarded form runs.
+// The per-pixel bounds of RaySkip (brickSplat, splatRect). Per marked brick: its span box,
+// widened by e voxels (e covers the rounding of the march's grid coordinates, which stays
+// below 4 2^-24 M voxels for a bound M on the magnitudes involved), is projected through
+// the inverse camera rotation. Only its part in front of the plane z = min_depth / 2 can
+// hold a sample (every sample lies at camera depth >= min_depth), so the corners there
+// and the box edges' crossings of the plane are projected; the perspective image of that
+// convex part is their hull, and the pixel rectangle around it, widened by one pixel,
+// takes the minimum of the box's distance from the camera (less 1e-5 of it) and the
+// maximum of its farthest point's distance (plus 1e-5 of it) in every 2 x 2 pixel cell
+// it touches, one cell per lane. A sample at parameter t is the point cam + dir t, at
+// distance t |dir|; a sample that is not far lies in some marked brick's widened box, in
+// front of the plane, so its pixel is in that brick's rectangle and its distance between
+// the two bounds.
+// ok = 0 (take every sample) unless the pose is finite and invertible and min_depth > 0.
+__global__ __launch_bounds__(256) void brickSplat(const unsigned char* __restrict__ bricks, int nb, VolParams p,
+                                                  PoseArg pa, Intr K, int W, int H, unsigned* __restrict__ dsafe,
+                                                  int* __restrict__ ok) {
+    const Rigid P = rigidFrom(pa);
+    const M33& R = P.R;
+    M33 A;  // adjugate of R
+    A.m[0][0] = R.m[1][1] * R.m[2][2] - R.m[1][2] * R.m[2][1];
+    A.m[0][1] = R.m[0][2] * R.m[2][1] - R.m[0][1] * R.m[2][2];
+    A.m[0][2] = R.m[0][1] * R.m[1][2] - R.m[0][2] * R.m[1][1];
+    A.m[1][0] = R.m[1][2] * R.m[2][0] - R.m[1][0] * R.m[2][2];
+    A.m[1][1] = R.m[0][0] * R.m[2][2] - R.m[0][2] * R.m[2][0];
+    A.m[1][2] = R.m[0][2] * R.m[1][0] - R.m[0][0] * R.m[1][2];
+    A.m[2][0] = R.m[1][0] * R.m[2][1] - R.m[1][1] * R.m[2][0];
+    A.m[2][1] = R.m[0][1] * R.m[2][0] - R.m[0][0] * R.m[2][1];
+    A.m[2][2] = R.m[0][0] * R.m[1][1] - R.m[0][1] * R.m[1][0];
+    const float det = R.m[0][0] * A.m[0][0] + R.m[0][1] * A.m[1][0] + R.m[0][2] * A.m[2][0];
+    bool valid = isfinite(det) && fabsf(det) > 1e-3f && p.min_depth > 0.0f && f3finite(P.t);
+    M33 Ri;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            Ri.m[r][c] = A.m[r][c] / det;
+            valid = valid && isfinite(Ri.m[r][c]);
+        }
+    if (blockIdx.x == 0 && threadIdx.x == 0) *ok = valid ? 1 : 0;
+    if (!valid) return;
+
+    // Wave w takes bricks kSplatPer w ... kSplatPer (w + 1) - 1, one lane each for the test
+    // and the projection; then all lanes splat each marked brick's cells in turn.
+    const int lane = threadIdx.x & 63;
+    const size_t bi = ((size_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * kSplatPer + (size_t)lane;
+    const bool marked = lane < kSplatPer && bi < (size_t)nb * nb * nb && bricks[bi] != 0;
+    if (__ballot(marked) == 0ull) return;  // whole wave
+    int cx0 = 0, cy0 = 0, w = 0, area = 0;
+    unsigned dlow = 0u, dhigh = 0u;
+    if (marked) splatRect(P, Ri, p, K, W, H, nb, bi, cx0, cy0, w, area, dlow, dhigh);
+    const int cw = (W + 1) >> 1;
+    for (unsigned long long m = __ballot(area > 0); m != 0ull; m &= m - 1ull) {
+        const int l = __builtin_ctzll(m);
+        const int bx0 = __builtin_amdgcn_readlane(cx0, l), by0 = __builtin_amdgcn_readlane(cy0, l);
+        const int bw = __builtin_amdgcn_readlane(w, l), ba = __builtin_amdgcn_readlane(area, l);
+        const unsigned bl = (unsigned)__builtin_amdgcn_readlane((int)dlow, l);
+        const unsigned bh = (unsigned)__builtin_amdgcn_readlane((int)dhigh, l);
+        for (int k = lane; k < ba; k += 64) {
+            unsigned* c = dsafe + 2 * ((size_t)(by0 + k / bw) * cw + (bx0 + k % bw));
+            atomicMin(c, bl);
+            atomicMax(c + 1, bh);
+        }
     }
 }
 
@@ -565,7 +594,7 @@ hipError_t launchSkipInit(unsigned* dsafe, size_t cells, hipStream_t st) {
 hipError_t launchBrickSplat(const unsigned char* bricks, const VolParams& p, const PoseArg& pa, Intr K, int W, int H,
                             unsigned* dsafe, int* ok, hipStream_t st) {
     const int nb = brickCount(p.res);
-    const unsigned blocks = (unsigned)(((size_t)nb * nb * nb + 3) / 4);
+    const unsigned blocks = (unsigned)(((size_t)nb * nb * nb + 4 * kSplatPer - 1) / (4 * kSplatPer));
     hipLaunchKernelGGL(brickSplat, dim3(blocks), dim3(256), 0, st, bricks, nb, p, pa, K, W, H, dsafe, ok);
     return hipGetLastError();
 }
