@@ -290,8 +290,9 @@ def rooflines(prof, nprof, n_upd, icp, s_v, W, H, res, u_ray=None):
             extra = {"touched_voxels": int(u_ray)}
         elif k == "icp_iter":
             # live vertex + normal per interior pixel, model vertex + normal per in-bounds
-            # projection, and the per-block partials (34 doubles x 256 blocks, written and
-            # read back) of every iteration that ran
+            # projection, and the per-block partials (34 doubles x at most 256 blocks, written
+            # and read back; 256 is the level-0 count and an upper bound for the coarser
+            # levels' 150 and 38) of every iteration that ran
             b = 24 * icp["pixels"] + 24 * icp["projections"] + 2 * 34 * 8 * 256 * icp["iterations_run"]
             extra = {"iterations_run": round(icp["iterations_run"], 2), "pixels": int(icp["pixels"]),
                      "projections": int(icp["projections"])}
