@@ -134,7 +134,7 @@ struct StoreF16 {
 // quotient for every x depends on b, and it fails where the quotient underflows). It
 // replaces the IEEE quotient x / b only after verifyKernel has compared the two on the
 // device for every float with 2^-100 <= |x| < 2^32 and found them equal (for the voxel
-// sizes and truncations 1 - 12 mm it holds). Smaller |x| takes the IEEE division; a
+// sizes and truncations 1, 2, 3, 4, 6 and 12 mm it holds). Smaller |x| takes the IEEE division; a
 // larger |x| only arises for a sample or distance that both forms reject or clamp (far
 // outside the volume, or sdf / trunc > 1 folded as 1).
 // ---------------------------------------------------------------------------
