@@ -148,27 +148,40 @@ __device__ __forceinline__ float2 boundsJoin(float2 a, float2 b) {
     return make_float2((a.x < 0.0f || b.x < 0.0f) ? -1.0f : fminf(a.x, b.x), fmaxf(a.y, b.y));
 }
 
-// One block per 64 x 64 pixel region: levels 1..6 of its cells, reduced in LDS.
-// Block (0, 0) also writes the frame's shared record of the item pipeline when sh is set.
+// One block per 64 x 64 pixel region: levels 1..6 of its cells, reduced in LDS. When sh
+// is set, one more block (the first of an extra row of the grid) writes the frame's
+// shared record of the item pipeline, beside the others rather than ahead of one of them.
 __global__ __launch_bounds__(256) void depthBoundsKernel(const float* __restrict__ depth, int W, int H, float min_d,
                                                          float max_d, float2* __restrict__ out, DepthBounds b,
                                                          IntShared* __restrict__ sh, VolParams p, PoseArg pa,
                                                          Intr K) {
-    if (sh && blockIdx.x == 0 && blockIdx.y == 0) itemSetup(sh, p, pa, K, W, H, max_d);
+    if (sh && blockIdx.y == gridDim.y - 1) {
+        if (blockIdx.x == 0) itemSetup(sh, p, pa, K, W, H, max_d);
+        return;
+    }
     __shared__ float2 cell[32][33];
     const int tid = threadIdx.x;
     const int X0 = blockIdx.x * 64, Y0 = blockIdx.y * 64;
-    // level 1: 32 x 32 cells of 2 x 2 pixels, 4 per thread
-    for (int k = tid; k < 32 * 32; k += 256) {
-        const int cx = k & 31, cy = k >> 5;
-        float2 acc = make_float2(__builtin_inff(), -__builtin_inff());
+    // level 1: 32 x 32 cells of 2 x 2 pixels, 4 per thread; the 16 reads issued together
+    float d[4][4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int k = tid + 256 * q, cx = k & 31, cy = k >> 5;
+#pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int x = X0 + 2 * cx + (j & 1), y = Y0 + 2 * cy + (j >> 1);
-            float2 pxb = make_float2(-1.0f, -__builtin_inff());
-            if (x < W && y < H) {
-                const float d = depth[(size_t)y * W + x];
-                if (d >= min_d && d <= max_d) pxb = make_float2(d, d);
-            }
+            d[q][j] = (x < W && y < H) ? depth[(size_t)y * W + x] : __builtin_nanf("");
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int k = tid + 256 * q, cx = k & 31, cy = k >> 5;
+        float2 acc = make_float2(__builtin_inff(), -__builtin_inff());
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            // outside the image (NaN) or outside the band: {-1, -inf}
+            const float dj = d[q][j];
+            const float2 pxb = (dj >= min_d && dj <= max_d) ? make_float2(dj, dj) : make_float2(-1.0f, -__builtin_inff());
             acc = j == 0 ? pxb : boundsJoin(acc, pxb);
         }
         cell[cy][cx] = acc;
@@ -1175,8 +1188,9 @@ static DepthBounds boundsFor(void* bounds, const float* depth, int W, int H, flo
         off += bnd.w[l] * bnd.h[l];
     }
     bnd.lvl = static_cast<const float2*>(bounds);
-    hipLaunchKernelGGL(depthBoundsKernel, dim3((W + 63) / 64, (H + 63) / 64), dim3(256), 0, st, depth, W, H, min_d,
-                       max_d, static_cast<float2*>(bounds), bnd, sh, p, pa, K);
+    // (with sh: one extra row of blocks, whose first block writes the frame record)
+    hipLaunchKernelGGL(depthBoundsKernel, dim3((W + 63) / 64, (H + 63) / 64 + (sh ? 1 : 0)), dim3(256), 0, st, depth, W,
+                       H, min_d, max_d, static_cast<float2*>(bounds), bnd, sh, p, pa, K);
     return bnd;
 }
 
