@@ -562,7 +562,7 @@ __global__ __launch_bounds__(256, 8) void integrateExact(typename S::Word* __res
 // Item pipeline of the exact integration (the default when the volume has its scratch).
 // The same per-voxel arithmetic as integrateExact, bit for bit, organised so that only
 // the voxels that can update are visited:
-//   * integrateSetup runs once per frame (depthBoundsKernel's first lane) into an
+//   * integrateSetup runs once per frame (one lane of depthBoundsKernel's extra block) into an
 //     IntShared record that every later kernel of the frame reads;
 //   * subtileClassKernel classifies sub-tiles of 32 columns x kSubRows rows at one y (a
 //     quarter of a tile's rows: a tighter depth range per unit, so more of them are free
