@@ -60,6 +60,43 @@ def test_raycast_exact_bitwise():
         assert np.array_equal(ng.view(np.uint32), no.view(np.uint32))
 
 
+def test_raycast_degenerate_poses_bitwise():
+    """Poses whose rays never reach a sample (a non-finite rotation or translation, a
+    camera far outside the volume) and a camera inside the volume: the FP32 raycast
+    against the oracle bit for bit (every non-finite pose is an all-miss image there), and
+    the FP16 + U16 form gives the same all-miss images for the non-finite poses."""
+    p = small_params(64, 0.016)
+    g = gpu_volume(p)
+    h = gpu_volume(p, fmt=N.FMT_F16_U16)
+    o = oracle_volume(p)
+    fr = frames(2)
+    for mm, T in fr:
+        d = oracle.depth_mm(mm)
+        o.integrate(d, sc.K640, T)
+        g.integrate(d, sc.K640, T)
+        h.integrate(d, sc.K640, T)
+    T0 = np.asarray(fr[1][1], np.float32)
+    cases = {}
+    for name, (r, c, val) in {"rot_nan": (0, 1, np.nan), "rot_inf": (2, 2, np.inf), "t_nan": (0, 3, np.nan),
+                              "t_inf": (2, 3, -np.inf), "t_far": (0, 3, 1.0e6)}.items():
+        T = T0.copy()
+        T[r, c] = val
+        cases[name] = T
+    inside = T0.copy()
+    inside[:3, 3] = [0.0, 0.0, 0.8]  # the sphere's centre, inside the volume
+    cases["inside"] = inside
+    for name, T in cases.items():
+        vg, ng = g.raycast(T, sc.K640)
+        vo, no = o.raycast(T, sc.K640, 640, 480)
+        say("degenerate pose", name, "oracle hits", int((vo[..., 2] != 0).sum()))
+        assert np.array_equal(vg.view(np.uint32), vo.view(np.uint32)), name
+        assert np.array_equal(ng.view(np.uint32), no.view(np.uint32)), name
+        if name not in ("t_far", "inside"):
+            assert not vo.any(), name
+            vh, nh = h.raycast(T, sc.K640)
+            assert not vh.any() and not nh.any(), name
+
+
 def test_live_pyramid_bitwise():
     from azu_amd.fusion import ICPTracker
     mm, T = frames(1)[0]
