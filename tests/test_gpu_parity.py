@@ -97,6 +97,38 @@ def test_raycast_degenerate_poses_bitwise():
             assert not vh.any() and not nh.any(), name
 
 
+def test_integrate_degenerate_poses_bitwise():
+    """A frame integrated at a degenerate pose (non-finite rotation or translation, a
+    camera far outside the volume) leaves the volume as the oracle leaves it, bit for bit
+    (FP32), and the FP16 + U16 volume unchanged where the pose is non-finite."""
+    p = small_params(64, 0.016)
+    g = gpu_volume(p)
+    h = gpu_volume(p, fmt=N.FMT_F16_U16)
+    o = oracle_volume(p)
+    mm, T = frames(1)[0]
+    d = oracle.depth_mm(mm)
+    o.integrate(d, sc.K640, T)
+    g.integrate(d, sc.K640, T)
+    h.integrate(d, sc.K640, T)
+    T0 = np.asarray(T, np.float32)
+    for name, (r, c, val) in {"rot_nan": (1, 0, np.nan), "rot_inf": (0, 0, -np.inf), "t_nan": (1, 3, np.nan),
+                              "t_inf": (0, 3, np.inf), "t_far": (2, 3, 1.0e6)}.items():
+        Tb = T0.copy()
+        Tb[r, c] = val
+        th0, wh0 = h.voxelData()
+        n_o, _ = o.integrate(d, sc.K640, Tb)
+        g.integrate(d, sc.K640, Tb)
+        h.integrate(d, sc.K640, Tb)
+        tg, wg = g.voxelData()
+        to, wo = o.get()
+        say("degenerate pose", name, "oracle updates", n_o)
+        assert np.array_equal(wg, wo), name
+        assert np.array_equal(tg.view(np.uint32), to.view(np.uint32)), name
+        if name != "t_far":
+            th1, wh1 = h.voxelData()
+            assert np.array_equal(wh1, wh0) and np.array_equal(th1.view(np.uint16), th0.view(np.uint16)), name
+
+
 def test_live_pyramid_bitwise():
     from azu_amd.fusion import ICPTracker
     mm, T = frames(1)[0]
